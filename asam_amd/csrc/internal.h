@@ -1,0 +1,160 @@
+// internal.h -- device data layout and kernel launchers of the gfx950 sound solver.
+//
+// Data layout in HBM (one block, N = nx*ny*nz cells, x fastest):
+//   stencil coefficients, one word per cell, structure-of-arrays:
+//     MultiTriT  (np1=1): kx,ky,kz (face coupling incl. face theta), vol, dut     = 5 words
+//     MultiTriTB (np1=2): kx,ky,kz (coupling without theta), tx,ty,tz (face theta),
+//                         gz (gravity coupling of the z-face above), vol, dut     = 9 words
+//     The face (c, c+e) of direction e is stored at cell c; wall faces hold 0.
+//   line factors (vertical-line LU of the z-tridiagonal): pl, pt, pw,
+//     np1*np1 words each per cell (2x2 blocks column-major [11,21,12,22]).
+//   vectors: p(1:np1, nx, ny, nz) as in the reference, component fastest.
+// Every array exists in FP64 and (after set-up) in FP32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace asam {
+
+constexpr double kEps = 1.0e-30;          // Kind_Mod Eps
+constexpr double kRd = 1004.0 - 717.0;    // Parameter_Mod Rd = Cpd - Cvd
+constexpr double kCpd = 1004.0;
+constexpr int kMaxM = 32;                 // largest (F)GMRES restart length
+constexpr int kRedBlocks = 16384;         // partial-sum slots per reduction quantity
+
+template <typename T>
+struct Coef {
+  T *kx = nullptr, *ky = nullptr, *kz = nullptr;
+  T *tx = nullptr, *ty = nullptr, *tz = nullptr, *gz = nullptr;
+  T *vol = nullptr, *dut = nullptr;
+  T *pl = nullptr, *pt = nullptr, *pw = nullptr;
+};
+
+struct Geom {
+  int nx, ny, nz, np1;
+  long N;      // cells
+  double fa;   // FacAnela
+};
+
+// Device-resident scalar state of one Krylov solve (written only by the
+// single-thread tails of the finalize kernels, read by every kernel).
+struct KState {
+  // BiCGStab
+  double rho, rho_old, alpha, omega, beta;
+  double rnorm0, rnorm, tol;
+  double sum[4];
+  int iter, maxiter, done, status;
+  int half, result, first, pad;
+  // (F)GMRES
+  double H[(kMaxM + 1) * kMaxM];  // H[row*kMaxM + col]
+  double cs[kMaxM], sn[kMaxM], bhat[kMaxM + 1], y[kMaxM], vscale[kMaxM + 1];
+  double gbeta, grho, grho0, gtol;
+  int m, step, nR, cycle_done;
+  int total, cycles, maxcycles, gstatus;
+  // mixed refinement
+  double outer_r0, outer_r, outer_tol, inner_tol;
+  int outer, inner_total, outer_done, pad2;
+};
+
+// ------------------------------------------------------------------ set-up
+void launch_weights(hipStream_t s, const Geom& g, const double* rho, const double* rth,
+                    const double* T, double* theta, double* dut);
+void launch_face_theta(hipStream_t s, const Geom& g, const double* vol, const double* theta,
+                       double* thu, double* thv, double* thw);
+// Face coefficients from areas, volumes, spacings and face theta (full face shapes).
+void launch_coef(hipStream_t s, const Geom& g, int grad_full, double beta0_dt, double grav,
+                 const double* FU, const double* FV, const double* FW, const double* vol,
+                 const double* dx, const double* dy, const double* dz, const double* thu,
+                 const double* thv, const double* thw, Coef<double>& c);
+void launch_factor(hipStream_t s, const Geom& g, Coef<double>& c);
+void launch_convert(hipStream_t s, const Geom& g, const Coef<double>& c64, Coef<float>& c32);
+
+// ------------------------------------------------------------------ apply
+// y = Op(x) = -A x (or -A^T x), optionally residual form y = b - Op(x_eff) with
+// x_eff = x (+ dx, written back to xw).  Partial sums: q0 = y.y, q1 = y.u1.
+enum { DOT_YY = 1, DOT_YU = 2 };
+struct ApplyReq {
+  int tr = 0;                  // transpose
+  int prec = 0;                // 0 FP64 coefficients, 1 FP32
+  int vx = 0, vo = 0, vu = 0;  // vector element types: 0 double, 1 float
+  const void* x = nullptr;
+  const float* dx = nullptr;   // mixed refinement correction (x must be double)
+  double* xw = nullptr;
+  const double* b = nullptr;   // residual form
+  void* y = nullptr;
+  void* y2 = nullptr;
+  const void* u = nullptr;
+  int dots = 0;
+  double* partial = nullptr;
+  const int* done = nullptr;
+};
+int apply_blocks(const Geom& g);
+void launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
+                  const ApplyReq& r);
+
+// ------------------------------------------------------------------ line solve
+// out = -M^-1 in (PSolve1 semantics) with in formed on the fly:
+//   LINE_PLAIN:  in = src * (*scale or 1)
+//   LINE_BICG_P: in = first ? r : r + beta*(p - omega*v); p := in
+//   LINE_BICG_S: in = r - alpha*v; r := in; partial q0 = in.in
+enum { LINE_PLAIN = 0, LINE_BICG_P = 1, LINE_BICG_S = 2 };
+struct LineReq {
+  int tr = 0, prec = 0, vt = 0;  // vt: vector type 0 double 1 float
+  int mode = LINE_PLAIN;
+  const void* src = nullptr;     // PLAIN source / r
+  void* p = nullptr;             // BICG_P: p (in/out); BICG_S: r (in/out)
+  const void* v = nullptr;
+  void* out = nullptr;
+  const double* scale = nullptr;
+  const KState* st = nullptr;
+  double* partial = nullptr;
+  const int* done = nullptr;
+  int use_lds = 1;
+};
+int line_blocks(const Geom& g);
+void launch_line(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
+                 const LineReq& r);
+
+// ------------------------------------------------------------------ BLAS-1 + reductions
+int blas_blocks(long n);
+// BiCGStab tail: x += alpha pHat + omega rHat; r -= omega t; q0 = r.r, q1 = rt.r.
+// With st->half set: x += alpha pHat only.
+void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,
+                        void* r, const void* t, const void* rt, const KState* st,
+                        double* partial);
+// (F)GMRES modified Gram-Schmidt step: w -= H[k-1][i]*vscale[k-1]*v_{k-1} (k>0); then
+// q0 = w.v_k (k<=i) or w.w (k==i+1).
+void launch_mgs(hipStream_t s, long n, int vt, void* w, const void* vprev, const void* vk,
+                int k, int i, const KState* st, double* partial, int chk_done = 0);
+// x += sum_{q<nR} y_q * Z_q  (Z_q = z vectors, FGMRES) with optional scale vscale (GMRES).
+void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs, int m,
+                         int use_vscale, const KState* st);
+void launch_dot(hipStream_t s, long n, const double* x, const double* y, double* partial);
+void launch_axpby(hipStream_t s, long n, double a, const double* x, double b, double* y);
+void launch_fill(hipStream_t s, long n, int vt, void* x, double v);
+void launch_copy_cvt(hipStream_t s, long n, int vt_dst, void* dst, int vt_src, const void* src);
+void launch_mass(hipStream_t s, const Geom& g, const double* vol, double* y, const double* x);
+void launch_theta_vjp(hipStream_t s, const Geom& g, const Coef<double>& c,
+                      const double* thfaces, double* grad, const double* mu, const double* x);
+
+// Finalize kernels: sum partial slots (deterministic order) then run the scalar logic.
+enum FinKind {
+  FIN_SUMS = 0,         // sums only -> st->sum[]
+  FIN_BICG_INIT,        // rnorm0, rho, iter=1
+  FIN_BICG_ALPHA,       // alpha = rho / (rt.v)
+  FIN_BICG_HALF,        // rnorm = |s|; half-step exit test
+  FIN_BICG_OMEGA,       // omega = (t.s)/(t.t)
+  FIN_BICG_END,         // rnorm = |r|; exit tests; rho, beta for the next iteration
+  FIN_GM_INIT,          // beta = |v0|; restart / exit bookkeeping
+  FIN_GM_H,             // H[k][i] = sum   (k <= i)
+  FIN_GM_LAST,          // H[i+1][i] = |w|, Givens, convergence test
+  FIN_DOT,              // store sum[0] to host-visible double (out)
+};
+void launch_finalize(hipStream_t s, int kind, int nq, int nblk, const double* partial,
+                     KState* st, int k, int i, double* out);
+// Back-substitution of the (F)GMRES least-squares problem (single thread).
+void launch_gm_backsolve(hipStream_t s, KState* st);
+// Mixed refinement control: after the FP64 residual of the refinement step.
+void launch_mixed_outer(hipStream_t s, int nblk, const double* partial, KState* st);
+
+}  // namespace asam

@@ -1,0 +1,783 @@
+// solver.hip -- C-ABI of the gfx950 sound stage solver: context, set-up, the
+// SolveSound drop-in (BiCGStab / FGMRES / GMRES, FP64 or FP32-inner with FP64
+// refinement), the adjoint pair, and device helpers.
+//
+// Krylov control flow stays on the device: every kernel of an iteration reads
+// the KState flags and returns at once after convergence, so the host enqueues
+// iterations in chunks and synchronises only between chunks; the iterates are
+// exactly those of a loop that stops at the first converged iteration.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/asam_sound.h"
+#include "internal.h"
+
+using namespace asam;
+
+struct asam_sound_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  asam_sound_grid grid{};
+  asam_sound_control ctl{};
+  Geom g{};
+  int ready = 0;
+  long n = 0;  // np1 * N
+  Coef<double> c64;
+  Coef<float> c32;
+  double* thfaces = nullptr;  // face theta, FU|FV|FW shapes concatenated
+  double* partial = nullptr;
+  KState* st = nullptr;
+  KState* st_host = nullptr;  // pinned
+  // FP64 work vectors
+  double *r = nullptr, *rt = nullptr, *p = nullptr, *ph = nullptr, *v = nullptr,
+         *rh = nullptr, *t = nullptr, *xd = nullptr, *bd = nullptr, *xalt = nullptr;
+  // FP32 work vectors (mixed)
+  float *r32 = nullptr, *rt32 = nullptr, *p32 = nullptr, *ph32 = nullptr, *v32 = nullptr,
+        *rh32 = nullptr, *t32 = nullptr, *d32 = nullptr;
+  // (F)GMRES basis
+  int gm_m = 0;
+  std::vector<double*> gv, gz;
+  double* gtmp = nullptr;
+  std::vector<void*> allocs;
+  int counts[8] = {0};
+  int last_iters = 4;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+#define HIPCHK(ctx, expr)                                                         \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);             \
+      return ASAM_E_HIP;                                                          \
+    }                                                                             \
+  } while (0)
+
+static int fail(asam_sound_ctx* c, int code, const char* msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename T>
+static int dalloc(asam_sound_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) {
+    c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return ASAM_E_NOMEM;
+  }
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+static void dfree(asam_sound_ctx* c, void* p) {
+  if (!p) return;
+  for (size_t i = 0; i < c->allocs.size(); ++i)
+    if (c->allocs[i] == p) {
+      c->allocs.erase(c->allocs.begin() + i);
+      break;
+    }
+  (void)hipFree(p);
+}
+
+extern "C" int asam_sound_default_control(asam_sound_control* ctl) {
+  if (!ctl) return ASAM_E_INVALID;
+  ctl->method = ASAM_METHOD_BICGSTAB;
+  ctl->step_gmres = 10;
+  ctl->precision = ASAM_PREC_FP64;
+  ctl->inner_tol = 1.0e-5;
+  ctl->max_inner = 200;
+  ctl->max_outer = 12;
+  return 0;
+}
+
+extern "C" const char* asam_sound_last_error(const asam_sound_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* grid) {
+  if (!out || !grid) return ASAM_E_INVALID;
+  *out = nullptr;
+  if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1 || (grid->np1 != 1 && grid->np1 != 2))
+    return ASAM_E_INVALID;
+  long N = (long)grid->nx * grid->ny * grid->nz;
+  if (N > (1L << 31)) return ASAM_E_INVALID;
+  {
+    // every reduction launch must fit the partial-sum slots
+    Geom tg{grid->nx, grid->ny, grid->nz, grid->np1, N, 1.0};
+    if (apply_blocks(tg) > kRedBlocks || line_blocks(tg) > kRedBlocks ||
+        (long)((grid->nx + 63) / 64) * ((grid->ny + 3) / 4) > kRedBlocks)
+      return ASAM_E_INVALID;
+  }
+  asam_sound_ctx* c = new asam_sound_ctx();
+  c->grid = *grid;
+  asam_sound_default_control(&c->ctl);
+  c->g.nx = grid->nx; c->g.ny = grid->ny; c->g.nz = grid->nz; c->g.np1 = grid->np1;
+  c->g.N = N; c->g.fa = grid->fac_anela;
+  c->n = N * grid->np1;
+  int rc = 0;
+  auto fin = [&](int code) { *out = c; return code; };
+  if (hipGetDevice(&c->device) != hipSuccess) { c->err = "no HIP device"; return fin(ASAM_E_HIP); }
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreate(&c->ev0));
+  HIPCHK(c, hipEventCreate(&c->ev1));
+  const long P = (long)grid->np1 * grid->np1 * N;
+  auto coef64 = [&](Coef<double>& k) -> int {
+    int r = 0;
+    r |= dalloc(c, &k.kx, N); r |= dalloc(c, &k.ky, N); r |= dalloc(c, &k.kz, N);
+    if (grid->np1 == 2) {
+      r |= dalloc(c, &k.tx, N); r |= dalloc(c, &k.ty, N); r |= dalloc(c, &k.tz, N);
+      r |= dalloc(c, &k.gz, N);
+    }
+    r |= dalloc(c, &k.vol, N); r |= dalloc(c, &k.dut, N);
+    r |= dalloc(c, &k.pl, P); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
+    return r;
+  };
+  auto coef32 = [&](Coef<float>& k) -> int {
+    int r = 0;
+    r |= dalloc(c, &k.kx, N); r |= dalloc(c, &k.ky, N); r |= dalloc(c, &k.kz, N);
+    if (grid->np1 == 2) {
+      r |= dalloc(c, &k.tx, N); r |= dalloc(c, &k.ty, N); r |= dalloc(c, &k.tz, N);
+      r |= dalloc(c, &k.gz, N);
+    }
+    r |= dalloc(c, &k.vol, N); r |= dalloc(c, &k.dut, N);
+    r |= dalloc(c, &k.pl, P); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
+    return r;
+  };
+  rc |= coef64(c->c64);
+  rc |= coef32(c->c32);
+  long nf = (long)(grid->nx + 1) * grid->ny * grid->nz + (long)grid->nx * (grid->ny + 1) * grid->nz +
+            (long)grid->nx * grid->ny * (grid->nz + 1);
+  rc |= dalloc(c, &c->thfaces, nf);
+  rc |= dalloc(c, &c->partial, 4L * kRedBlocks);
+  rc |= dalloc(c, &c->st, 1);
+  const long n = c->n;
+  double** v64[] = {&c->r, &c->rt, &c->p, &c->ph, &c->v, &c->rh, &c->t, &c->xd, &c->bd, &c->xalt};
+  for (double** q : v64) rc |= dalloc(c, q, n);
+  float** v32[] = {&c->r32, &c->rt32, &c->p32, &c->ph32, &c->v32, &c->rh32, &c->t32, &c->d32};
+  for (float** q : v32) rc |= dalloc(c, q, n);
+  if (rc) return fin(ASAM_E_NOMEM);
+  HIPCHK(c, hipHostMalloc((void**)&c->st_host, sizeof(KState), hipHostMallocDefault));
+  memset(c->st_host, 0, sizeof(KState));
+  return fin(0);
+}
+
+extern "C" int asam_sound_destroy(asam_sound_ctx* c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  if (c->st_host) (void)hipHostFree(c->st_host);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int asam_sound_set_control(asam_sound_ctx* c, const asam_sound_control* ctl) {
+  if (!c || !ctl) return ASAM_E_INVALID;
+  if (ctl->method < 0 || ctl->method > 2) return fail(c, ASAM_E_INVALID, "method");
+  if (ctl->step_gmres < 1 || ctl->step_gmres > kMaxM) return fail(c, ASAM_E_INVALID, "step_gmres");
+  if (ctl->precision < 0 || ctl->precision > 1) return fail(c, ASAM_E_INVALID, "precision");
+  if (ctl->precision == ASAM_PREC_MIXED && ctl->method != ASAM_METHOD_BICGSTAB)
+    return fail(c, ASAM_E_INVALID, "mixed precision runs the BiCGStab inner solver");
+  c->ctl = *ctl;
+  return 0;
+}
+
+// ------------------------------------------------------------------ set-up
+static int setup_common(asam_sound_ctx* c, const double* FU, const double* FV, const double* FW,
+                        const double* V, const double* dx, const double* dy, const double* dz,
+                        const double* rho, const double* rth, const double* T,
+                        const double* DTU, const double* DTV, const double* DTW,
+                        const double* DUT) {
+  const Geom& g = c->g;
+  const long N = g.N;
+  const long nfu = (long)(g.nx + 1) * g.ny * g.nz, nfv = (long)g.nx * (g.ny + 1) * g.nz,
+             nfw = (long)g.nx * g.ny * (g.nz + 1);
+  for (long q = 0; q < N; ++q)
+    if (!(V[q] >= 0.0) || !isfinite(V[q])) return fail(c, ASAM_E_NAN, "VolC must be finite, >= 0");
+  double *dFU, *dFV, *dFW, *dV, *ddx, *ddy, *ddz, *tmp = nullptr;
+  int rc = 0;
+  rc |= dalloc(c, &dFU, nfu); rc |= dalloc(c, &dFV, nfv); rc |= dalloc(c, &dFW, nfw);
+  rc |= dalloc(c, &dV, N); rc |= dalloc(c, &ddx, g.nx); rc |= dalloc(c, &ddy, g.ny);
+  rc |= dalloc(c, &ddz, g.nz);
+  if (rc) return ASAM_E_NOMEM;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(dFU, FU, nfu * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dFV, FV, nfv * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dFW, FW, nfw * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dV, V, N * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(ddx, dx, g.nx * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(ddy, dy, g.ny * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(ddz, dz, g.nz * 8, hipMemcpyHostToDevice, s));
+  double* thu = c->thfaces;
+  double* thv = thu + nfu;
+  double* thw = thv + nfv;
+  if (rho) {
+    double *drho, *drth, *dT;
+    rc |= dalloc(c, &drho, N); rc |= dalloc(c, &drth, N); rc |= dalloc(c, &dT, N);
+    rc |= dalloc(c, &tmp, N);
+    if (rc) return ASAM_E_NOMEM;
+    HIPCHK(c, hipMemcpyAsync(drho, rho, N * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(drth, rth, N * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(dT, T, N * 8, hipMemcpyHostToDevice, s));
+    launch_weights(s, g, drho, drth, dT, tmp, c->c64.dut);
+    launch_face_theta(s, g, dV, tmp, thu, thv, thw);
+    HIPCHK(c, hipStreamSynchronize(s));
+    dfree(c, drho); dfree(c, drth); dfree(c, dT); dfree(c, tmp);
+  } else {
+    HIPCHK(c, hipMemcpyAsync(thu, DTU, nfu * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(thv, DTV, nfv * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(thw, DTW, nfw * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->c64.dut, DUT, N * 8, hipMemcpyHostToDevice, s));
+  }
+  launch_coef(s, g, c->grid.grad_full, c->grid.beta0_dt, c->grid.grav, dFU, dFV, dFW, dV, ddx,
+              ddy, ddz, thu, thv, thw, c->c64);
+  launch_factor(s, g, c->c64);
+  launch_convert(s, g, c->c64, c->c32);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(s));
+  dfree(c, dFU); dfree(c, dFV); dfree(c, dFW); dfree(c, dV); dfree(c, ddx); dfree(c, ddy);
+  dfree(c, ddz);
+  c->ready = 1;
+  return 0;
+}
+
+extern "C" int asam_sound_setup(asam_sound_ctx* c, const double* FU, const double* FV,
+                                const double* FW, const double* V, const double* dx,
+                                const double* dy, const double* dz, const double* rho,
+                                const double* rth, const double* T) {
+  if (!c) return ASAM_E_INVALID;
+  if (!FU || !FV || !FW || !V || !dx || !dy || !dz || !rho || !rth || !T)
+    return fail(c, ASAM_E_INVALID, "null input");
+  HIPCHK(c, hipSetDevice(c->device));
+  return setup_common(c, FU, FV, FW, V, dx, dy, dz, rho, rth, T, nullptr, nullptr, nullptr,
+                      nullptr);
+}
+
+extern "C" int asam_sound_setup_weights(asam_sound_ctx* c, const double* FU, const double* FV,
+                                        const double* FW, const double* V, const double* dx,
+                                        const double* dy, const double* dz, const double* DTU,
+                                        const double* DTV, const double* DTW,
+                                        const double* DUT) {
+  if (!c) return ASAM_E_INVALID;
+  if (!FU || !FV || !FW || !V || !dx || !dy || !dz || !DTU || !DTV || !DTW || !DUT)
+    return fail(c, ASAM_E_INVALID, "null input");
+  HIPCHK(c, hipSetDevice(c->device));
+  return setup_common(c, FU, FV, FW, V, dx, dy, dz, nullptr, nullptr, nullptr, DTU, DTV, DTW,
+                      DUT);
+}
+
+// ------------------------------------------------------------------ enqueue helpers
+namespace {
+
+struct Enq {
+  asam_sound_ctx* c;
+  hipStream_t s;
+  int tr;
+  explicit Enq(asam_sound_ctx* cc, int t) : c(cc), s(cc->stream), tr(t) {}
+
+  void apply(int prec, int vx, int vo, int vu, const void* x, void* y, const double* b,
+             void* y2, const void* u, int dots, const int* done, const float* dx = nullptr,
+             double* xw = nullptr) {
+    ApplyReq r;
+    r.tr = tr; r.prec = prec; r.vx = vx; r.vo = vo; r.vu = vu; r.x = x; r.y = y; r.b = b;
+    r.y2 = y2; r.u = u; r.dots = dots; r.partial = c->partial; r.done = done; r.dx = dx;
+    r.xw = xw;
+    launch_apply(s, c->g, c->c64, c->c32, r);
+    c->counts[0]++;
+  }
+  void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
+            const double* scale, const int* done) {
+    LineReq r;
+    r.tr = tr; r.prec = prec; r.vt = prec; r.mode = mode; r.src = src; r.p = p; r.v = v;
+    r.out = out; r.scale = scale; r.st = c->st; r.partial = c->partial; r.done = done;
+    launch_line(s, c->g, c->c64, c->c32, r);
+    c->counts[1]++;
+  }
+  void fin(int kind, int nq, int nblk, int k = 0, int i = 0) {
+    launch_finalize(s, kind, nq, nblk, c->partial, c->st, k, i, nullptr);
+    c->counts[3]++;
+  }
+};
+
+int sync_state(asam_sound_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int upload_state(asam_sound_ctx* c) {
+  HIPCHK(c, hipMemcpyAsync(c->st, c->st_host, sizeof(KState), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// One BiCGStab iteration (SpMatrix4CellPar_Mod.f90:280-337) as 5 fused kernels + 4 tails.
+// prec/vt: 0 FP64, 1 FP32 vectors and coefficients (dots always FP64).
+template <typename VT>
+void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v, VT* rh, VT* t) {
+  asam_sound_ctx* c = e.c;
+  KState* st = c->st;
+  const Geom& g = c->g;
+  const int vt = prec;
+  // p = r + beta (p - omega v) (p = r at iter 1); pHat = PSolve(p)
+  e.line(prec, LINE_BICG_P, r, p, v, ph, nullptr, &st->done);
+  // v = MatVec(pHat); alpha = rho / (rTilde.v)
+  e.apply(prec, vt, vt, vt, ph, v, nullptr, nullptr, rt, DOT_YU, &st->done);
+  e.fin(FIN_BICG_ALPHA, 2, apply_blocks(g));
+  // s = r - alpha v (into r); |s|; rHat = PSolve(s)
+  e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done);
+  e.fin(FIN_BICG_HALF, 1, line_blocks(g));
+  // t = MatVec(rHat); omega = (t.s)/(t.t)
+  e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->pad);
+  e.fin(FIN_BICG_OMEGA, 2, apply_blocks(g));
+  // x += alpha pHat + omega rHat; r = s - omega t; |r|, rTilde.r
+  launch_bicg_update(e.s, c->n, vt, x, ph, rh, r, t, rt, st, c->partial);
+  c->counts[2]++;
+  e.fin(FIN_BICG_END, 2, blas_blocks(c->n));
+}
+
+// Runs BiCGStab iterations until the device reports done; returns 0 or error.
+template <typename VT>
+int bicg_loop(Enq& e, int prec, int maxiter, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v, VT* rh,
+              VT* t, int* iters_out) {
+  asam_sound_ctx* c = e.c;
+  int issued = 0;
+  int chunk = c->last_iters > 0 ? c->last_iters : 4;
+  if (chunk > maxiter) chunk = maxiter;
+  while (issued < maxiter) {
+    int q = chunk;
+    if (issued + q > maxiter) q = maxiter - issued;
+    for (int k = 0; k < q; ++k) bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
+    issued += q;
+    int rc = sync_state(c);
+    if (rc) return rc;
+    if (c->st_host->done) break;
+    chunk = 2;
+  }
+  *iters_out = issued;
+  return 0;
+}
+
+}  // namespace
+
+static int solve_bicg_fp64(asam_sound_ctx* c, int tr, double* x, const double* b, int* maxiter,
+                           double* tol, asam_sound_info* info) {
+  Enq e(c, tr);
+  KState* hs = c->st_host;
+  memset(hs, 0, sizeof(KState));
+  hs->maxiter = *maxiter;
+  hs->tol = *tol;
+  int rc = upload_state(c);
+  if (rc) return rc;
+  // r = b - MatVec(x); rTilde = r; rNorm0 = |r|
+  e.apply(0, 0, 0, 0, x, c->r, b, c->rt, nullptr, DOT_YY, nullptr);
+  e.fin(FIN_BICG_INIT, 1, apply_blocks(c->g));
+  int issued = 0;
+  if (*maxiter > 0) {
+    rc = bicg_loop<double>(e, 0, *maxiter, x, c->r, c->rt, c->p, c->ph, c->v, c->rh, c->t,
+                           &issued);
+    if (rc) return rc;
+  } else {
+    rc = sync_state(c);
+    if (rc) return rc;
+  }
+  const KState& s = *hs;
+  int res = s.done ? s.result : 0;
+  int it = res < 0 ? -res : (s.status == 1 ? res / 2 : res);
+  if (s.status == 2) it = res;
+  c->last_iters = it > 0 ? it : 1;
+  *maxiter = res;
+  *tol = s.rnorm;
+  if (info) {
+    info->iterations = it;
+    info->restarts = 0;
+    info->status = s.status == 0 ? ASAM_CONVERGED : (s.status == 2 ? ASAM_BREAKDOWN : ASAM_NOT_CONVERGED);
+    info->abs_residual = s.rnorm;
+    info->rel_residual = s.rnorm0 > 0 ? s.rnorm / s.rnorm0 : 0.0;
+  }
+  return 0;
+}
+
+static int ensure_gmres(asam_sound_ctx* c, int m) {
+  if (c->gm_m >= m) return 0;
+  for (double* q : c->gv) dfree(c, q);
+  for (double* q : c->gz) dfree(c, q);
+  c->gv.assign(m + 1, nullptr);
+  c->gz.assign(m, nullptr);
+  int rc = 0;
+  for (int i = 0; i <= m; ++i) rc |= dalloc(c, &c->gv[i], c->n);
+  for (int i = 0; i < m; ++i) rc |= dalloc(c, &c->gz[i], c->n);
+  if (!c->gtmp) rc |= dalloc(c, &c->gtmp, c->n);
+  if (rc) return ASAM_E_NOMEM;
+  c->gm_m = m;
+  return 0;
+}
+
+// FGMRES(m) (Iter_Mod.f90:121-199) or left-preconditioned GMRES(m) (:32-119), FP64.
+static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, const double* b,
+                            int maxcycles, double tol, asam_sound_info* info) {
+  const int m = c->ctl.step_gmres;
+  int rc = ensure_gmres(c, m);
+  if (rc) return rc;
+  Enq e(c, tr);
+  KState* hs = c->st_host;
+  memset(hs, 0, sizeof(KState));
+  hs->m = m;
+  hs->gtol = tol;
+  hs->maxcycles = maxcycles;
+  rc = upload_state(c);
+  if (rc) return rc;
+  KState* st = c->st;
+  const Geom& g = c->g;
+  const long n = c->n;
+  for (int cyc = 0; cyc < maxcycles; ++cyc) {
+    if (flexible) {
+      e.apply(0, 0, 0, 0, x, c->gv[0], b, nullptr, nullptr, DOT_YY, &st->done);
+      e.fin(FIN_GM_INIT, 1, apply_blocks(g));
+    } else {
+      e.apply(0, 0, 0, 0, x, c->gtmp, b, nullptr, nullptr, 0, &st->done);
+      e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[0], nullptr, &st->done);
+      launch_mgs(e.s, n, 0, c->gv[0], nullptr, nullptr, 0, -1, st, c->partial, 1);
+      c->counts[2]++;
+      e.fin(FIN_GM_INIT, 1, blas_blocks(n));
+    }
+    for (int i = 0; i < m; ++i) {
+      if (flexible) {
+        e.line(0, LINE_PLAIN, c->gv[i], nullptr, nullptr, c->gz[i], &st->vscale[i],
+               &st->cycle_done);
+        e.apply(0, 0, 0, 0, c->gz[i], c->gv[i + 1], nullptr, nullptr, c->gv[0], DOT_YU,
+                &st->cycle_done);
+        e.fin(FIN_GM_H, 2, apply_blocks(g), 0, i);
+      } else {
+        e.apply(0, 0, 0, 0, c->gv[i], c->gtmp, nullptr, nullptr, nullptr, 0, &st->cycle_done);
+        e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[i + 1], &st->vscale[i],
+               &st->cycle_done);
+        launch_mgs(e.s, n, 0, c->gv[i + 1], nullptr, c->gv[0], 0, i, st, c->partial);
+        c->counts[2]++;
+        e.fin(FIN_GM_H, 1, blas_blocks(n), 0, i);
+      }
+      for (int k = 1; k <= i; ++k) {
+        launch_mgs(e.s, n, 0, c->gv[i + 1], c->gv[k - 1], c->gv[k], k, i, st, c->partial);
+        c->counts[2]++;
+        e.fin(FIN_GM_H, 1, blas_blocks(n), k, i);
+      }
+      launch_mgs(e.s, n, 0, c->gv[i + 1], c->gv[i], nullptr, i + 1, i, st, c->partial);
+      c->counts[2]++;
+      e.fin(FIN_GM_LAST, 1, blas_blocks(n), 0, i);
+    }
+    launch_gm_backsolve(e.s, st);
+    if (flexible)
+      launch_gmres_update(e.s, n, 0, x, (void* const*)c->gz.data(), m, 0, st);
+    else
+      launch_gmres_update(e.s, n, 0, x, (void* const*)c->gv.data(), m, 1, st);
+    c->counts[2]++;
+    rc = sync_state(c);
+    if (rc) return rc;
+    if (hs->done) break;
+  }
+  if (info) {
+    info->iterations = hs->total;
+    info->restarts = hs->cycles;
+    info->status = (hs->grho <= tol * hs->grho0) ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
+    info->abs_residual = hs->grho;
+    info->rel_residual = hs->grho0 > 0 ? hs->grho / hs->grho0 : 0.0;
+  }
+  return 0;
+}
+
+// FP32-inner BiCGStab with FP64 iterative refinement.
+static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, double tol,
+                       asam_sound_info* info) {
+  Enq e(c, tr);
+  KState* hs = c->st_host;
+  memset(hs, 0, sizeof(KState));
+  hs->outer_tol = tol;
+  hs->inner_tol = c->ctl.inner_tol;
+  hs->maxiter = c->ctl.max_inner;
+  int rc = upload_state(c);
+  if (rc) return rc;
+  KState* st = c->st;
+  double* xcur = x;
+  double* xnext = (x == c->xalt) ? c->xd : c->xalt;
+  int inner_total = 0, outer = 0;
+  bool converged = false;
+  for (outer = 0; outer <= c->ctl.max_outer; ++outer) {
+    if (outer == 0) {
+      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr);
+    } else {
+      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, c->d32, xnext);
+      double* tmp = xcur; xcur = xnext; xnext = tmp;
+    }
+    launch_mixed_outer(e.s, apply_blocks(c->g), c->partial, st);
+    c->counts[3]++;
+    rc = sync_state(c);
+    if (rc) return rc;
+    if (hs->outer_done) { converged = true; break; }
+    if (outer == c->ctl.max_outer) break;
+    launch_fill(e.s, c->n, 1, c->d32, 0.0);
+    int issued = 0;
+    rc = bicg_loop<float>(e, 1, c->ctl.max_inner, c->d32, c->r32, c->rt32, c->p32, c->ph32,
+                          c->v32, c->rh32, c->t32, &issued);
+    if (rc) return rc;
+    int res = hs->result;
+    int it = res < 0 ? -res : (hs->status == 1 ? res / 2 : res);
+    inner_total += it;
+  }
+  if (xcur != x) HIPCHK(c, hipMemcpyAsync(x, xcur, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (info) {
+    info->iterations = inner_total;
+    info->restarts = outer;
+    info->status = converged ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
+    info->abs_residual = hs->outer_r;
+    info->rel_residual = hs->outer_r0 > 0 ? hs->outer_r / hs->outer_r0 : 0.0;
+  }
+  return 0;
+}
+
+static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double* b, int* maxiter,
+                             double* tol, asam_sound_info* info) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "solve before setup");
+  if (!x || !b || !maxiter || !tol) return fail(c, ASAM_E_INVALID, "null argument");
+  if (!(*tol >= 0.0)) return fail(c, ASAM_E_INVALID, "tol");
+  HIPCHK(c, hipSetDevice(c->device));
+  memset(c->counts, 0, sizeof(c->counts));
+  asam_sound_info tmp;
+  memset(&tmp, 0, sizeof tmp);
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  int rc;
+  if (c->ctl.precision == ASAM_PREC_MIXED) {
+    rc = solve_mixed(c, tr, x, b, *tol, &tmp);
+  } else if (c->ctl.method == ASAM_METHOD_BICGSTAB) {
+    rc = solve_bicg_fp64(c, tr, x, b, maxiter, tol, &tmp);
+  } else {
+    rc = solve_gmres_fp64(c, tr, c->ctl.method == ASAM_METHOD_FGMRES, x, b, *maxiter, *tol,
+                          &tmp);
+  }
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  tmp.ms = ms;
+  HIPCHK(c, hipGetLastError());
+  if (!isfinite(tmp.abs_residual)) {
+    if (info) *info = tmp;
+    return fail(c, ASAM_E_NAN, "non-finite residual");
+  }
+  if (info) *info = tmp;
+  return 0;
+}
+
+extern "C" int asam_sound_solve_device(asam_sound_ctx* c, double* x, const double* b,
+                                       int* maxiter, double* tol, asam_sound_info* info) {
+  return solve_device_impl(c, 0, x, b, maxiter, tol, info);
+}
+
+extern "C" int asam_sound_solve_T_device(asam_sound_ctx* c, double* x, const double* b,
+                                         int* maxiter, double* tol, asam_sound_info* info) {
+  return solve_device_impl(c, 1, x, b, maxiter, tol, info);
+}
+
+static int host_solve(asam_sound_ctx* c, int tr, double* x, const double* b, int* maxiter,
+                      double* tol, asam_sound_info* info) {
+  if (!c) return ASAM_E_INVALID;
+  if (!x || !b) return fail(c, ASAM_E_INVALID, "null argument");
+  if (!c->ready) return fail(c, ASAM_E_STATE, "solve before setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(c->xd, x, c->n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->bd, b, c->n * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = solve_device_impl(c, tr, c->xd, c->bd, maxiter, tol, info);
+  if (rc && rc != ASAM_E_NAN) return rc;
+  HIPCHK(c, hipMemcpyAsync(x, c->xd, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rc;
+}
+
+extern "C" int asam_sound_solve(asam_sound_ctx* c, double* x, const double* b, int* maxiter,
+                                double* tol, asam_sound_info* info) {
+  return host_solve(c, 0, x, b, maxiter, tol, info);
+}
+
+extern "C" int asam_sound_solve_T(asam_sound_ctx* c, double* x, const double* b, int* maxiter,
+                                  double* tol, asam_sound_info* info) {
+  return host_solve(c, 1, x, b, maxiter, tol, info);
+}
+
+// ------------------------------------------------------------------ operator pieces
+extern "C" int asam_sound_apply_device(asam_sound_ctx* c, double* y, const double* x, int tr) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "apply before setup");
+  if (!x || !y || x == y) return fail(c, ASAM_E_INVALID, "x, y must be distinct buffers");
+  HIPCHK(c, hipSetDevice(c->device));
+  Enq e(c, tr ? 1 : 0);
+  e.apply(0, 0, 0, 0, x, y, nullptr, nullptr, nullptr, 0, nullptr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int asam_sound_psolve_device(asam_sound_ctx* c, double* y, const double* x, int tr) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "psolve before setup");
+  if (!x || !y) return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  Enq e(c, tr ? 1 : 0);
+  e.line(0, LINE_PLAIN, x, nullptr, nullptr, y, nullptr, nullptr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+static int host_unary(asam_sound_ctx* c, double* y, const double* x, int tr, int which) {
+  if (!c) return ASAM_E_INVALID;
+  if (!x || !y) return fail(c, ASAM_E_INVALID, "null argument");
+  if (!c->ready) return fail(c, ASAM_E_STATE, "before setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(c->xd, x, c->n * 8, hipMemcpyHostToDevice, c->stream));
+  int rc = which == 0 ? asam_sound_apply_device(c, c->bd, c->xd, tr)
+                      : asam_sound_psolve_device(c, c->bd, c->xd, tr);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(y, c->bd, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int asam_sound_apply(asam_sound_ctx* c, double* y, const double* x) {
+  return host_unary(c, y, x, 0, 0);
+}
+extern "C" int asam_sound_apply_T(asam_sound_ctx* c, double* y, const double* x) {
+  return host_unary(c, y, x, 1, 0);
+}
+extern "C" int asam_sound_psolve(asam_sound_ctx* c, double* y, const double* x) {
+  return host_unary(c, y, x, 0, 1);
+}
+extern "C" int asam_sound_psolve_T(asam_sound_ctx* c, double* y, const double* x) {
+  return host_unary(c, y, x, 1, 1);
+}
+
+extern "C" int asam_sound_theta_vjp_device(asam_sound_ctx* c, double* grad, const double* mu,
+                                           const double* x) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "before setup");
+  if (!grad || !mu || !x) return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  launch_theta_vjp(c->stream, c->g, c->c64, c->thfaces, grad, mu, x);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------------ device helpers
+extern "C" int asam_dev_alloc(asam_sound_ctx* c, void** ptr, size_t bytes) {
+  if (!c || !ptr) return ASAM_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMalloc(ptr, bytes ? bytes : 1));
+  return 0;
+}
+extern "C" int asam_dev_free(asam_sound_ctx* c, void* ptr) {
+  if (!c) return ASAM_E_INVALID;
+  if (ptr) HIPCHK(c, hipFree(ptr));
+  return 0;
+}
+extern "C" int asam_dev_upload(asam_sound_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || !dst || !src) return ASAM_E_INVALID;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+extern "C" int asam_dev_download(asam_sound_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c || !dst || !src) return ASAM_E_INVALID;
+  HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+extern "C" int asam_dev_memset(asam_sound_ctx* c, void* dst, int value, size_t bytes) {
+  if (!c || !dst) return ASAM_E_INVALID;
+  HIPCHK(c, hipMemsetAsync(dst, value, bytes, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+extern "C" int asam_dev_axpby(asam_sound_ctx* c, size_t n, double a, const double* x, double b,
+                              double* y) {
+  if (!c || !x || !y) return ASAM_E_INVALID;
+  launch_axpby(c->stream, (long)n, a, x, b, y);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+extern "C" int asam_dev_dot(asam_sound_ctx* c, size_t n, const double* x, const double* y,
+                            double* dot) {
+  if (!c || !x || !y || !dot) return ASAM_E_INVALID;
+  launch_dot(c->stream, (long)n, x, y, c->partial);
+  launch_finalize(c->stream, FIN_SUMS, 1, blas_blocks((long)n), c->partial, c->st, 0, 0, nullptr);
+  int rc = sync_state(c);
+  if (rc) return rc;
+  *dot = c->st_host->sum[0];
+  return 0;
+}
+extern "C" int asam_dev_mass(asam_sound_ctx* c, double* y, const double* x) {
+  if (!c || !x || !y) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "before setup");
+  launch_mass(c->stream, c->g, c->c64.vol, y, x);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------------ measurement
+extern "C" int asam_sound_time_kernel(asam_sound_ctx* c, int which, int fp32, int reps,
+                                      double* ms) {
+  if (!c || !ms || reps < 1) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "before setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  Enq e(c, 0);
+  const int prec = fp32 ? 1 : 0;
+  void* x = fp32 ? (void*)c->p32 : (void*)c->p;
+  void* y = fp32 ? (void*)c->v32 : (void*)c->v;
+  launch_fill(c->stream, c->n, prec, x, 1.0);
+  launch_fill(c->stream, c->n, prec, y, 0.5);
+  launch_fill(c->stream, c->n, prec, fp32 ? (void*)c->r32 : (void*)c->r, 0.25);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto once = [&]() {
+    if (which == 0) e.apply(prec, prec, prec, prec, x, y, nullptr, nullptr, nullptr, 0, nullptr);
+    else if (which == 1) e.line(prec, LINE_PLAIN, x, nullptr, nullptr, y, nullptr, nullptr);
+    else {
+      memset(c->st_host, 0, sizeof(KState));
+      launch_bicg_update(c->stream, c->n, prec, x, y, y, fp32 ? (void*)c->r32 : (void*)c->r, y,
+                         y, c->st, c->partial);
+    }
+  };
+  if (which == 2) {
+    memset(c->st_host, 0, sizeof(KState));
+    int rc = upload_state(c);
+    if (rc) return rc;
+  }
+  once();
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int q = 0; q < reps; ++q) once();
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float t = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  HIPCHK(c, hipGetLastError());
+  *ms = (double)t / reps;
+  return 0;
+}
+
+extern "C" int asam_sound_last_counts(const asam_sound_ctx* c, int counts[8]) {
+  if (!c || !counts) return ASAM_E_INVALID;
+  for (int q = 0; q < 8; ++q) counts[q] = c->counts[q];
+  return 0;
+}

@@ -1,0 +1,295 @@
+"""Seeded synthetic inputs for the sound stage solve.
+
+Generates the geometry (cut-cell face areas and volumes above a Witch-of-Agnesi
+ridge), the thermodynamic background (hydrostatic theta=300 K plus a Gaussian
+warm bubble) and the right-hand side b ~ N(0,1) that the caller would hand to
+``SolveSound``.  Everything is a pure function of the configuration and the
+seed, so the CPU oracle and the GPU library see bit-identical inputs.
+
+Array layout follows the reference's Fortran column-major arrays: x fastest,
+then y, then z.  In numpy (C order) a cell array therefore has shape
+``(nz, ny, nx)``; face arrays carry one extra entry along their normal axis:
+``FU`` (nz, ny, nx+1), ``FV`` (nz, ny+1, nx), ``FW`` (nz+1, ny, nx), matching
+``FU(ix0:ix1, iy0+1:iy1, iz0+1:iz1)`` etc. in ``SRC/MODULE/Domain_Mod.f90``.
+
+Physical constants are the reference's ``SRC/MODULE/Parameter_Mod.f90:21-23,39,67``
+(Cpd=1004, Cvd=717, Rd=Cpd-Cvd, p0=1e5, Grav=9.81).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+CPD = 1004.0
+CVD = 717.0
+RD = CPD - CVD
+P0 = 1.0e5
+GRAV = 9.81
+GAMMA_ROS = 0.4358665  # Rosenbrock gamma used as beta0
+
+MASK64 = (1 << 64) - 1
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+# ---------------------------------------------------------------------------
+# Counter-based RNG: SplitMix64 finaliser on (seed + n*golden), Box-Muller.
+# ---------------------------------------------------------------------------
+def splitmix64(counter: np.ndarray, seed: int) -> np.ndarray:
+    """SplitMix64 output for the counters ``counter`` (uint64 array)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & MASK64) + (counter.astype(np.uint64) + np.uint64(1)) * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(n: int, seed: int, offset: int = 0) -> np.ndarray:
+    """n uniforms in (0, 1]: 53-bit mantissa from SplitMix64, counters offset..offset+n-1."""
+    c = np.arange(offset, offset + n, dtype=np.uint64)
+    z = splitmix64(c, seed)
+    return ((z >> np.uint64(11)).astype(np.float64) + 1.0) * (1.0 / 9007199254740992.0)
+
+
+def normal(n: int, seed: int) -> np.ndarray:
+    """n standard normals via Box-Muller (cosine branch) on uniform pairs (2i, 2i+1)."""
+    u = uniform01(2 * n, seed)
+    u1 = u[0::2]
+    u2 = u[1::2]
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+# ---------------------------------------------------------------------------
+# Geometry: Witch-of-Agnesi ridge h(x) = H / (1 + ((x - xc)/a)^2), uniform in y.
+# ---------------------------------------------------------------------------
+def _agnesi_int(x, H, a, xc):
+    """Antiderivative of the Agnesi profile: H a atan((x-xc)/a)."""
+    return H * a * np.arctan((x - xc) / a)
+
+
+def _int_min_h(c, x0, x1, H, a, xc):
+    """Integral over [x0,x1] of min(h(x), c), vectorised over arrays c, x0, x1."""
+    c = np.asarray(c, dtype=np.float64)
+    full = _agnesi_int(x1, H, a, xc) - _agnesi_int(x0, H, a, xc)
+    out = np.array(full, dtype=np.float64, copy=True)
+    # Where 0 < c < H the profile exceeds c on |x - xc| < s.
+    inner = (c > 0.0) & (c < H)
+    s = np.where(inner, a * np.sqrt(np.maximum(H / np.where(c > 0, c, 1.0) - 1.0, 0.0)), 0.0)
+    lo = np.maximum(x0, xc - s)
+    hi = np.minimum(x1, xc + s)
+    overlap = inner & (hi > lo)
+    excess = (_agnesi_int(hi, H, a, xc) - _agnesi_int(lo, H, a, xc)) - c * (hi - lo)
+    out = np.where(overlap, out - excess, out)
+    out = np.where(c <= 0.0, np.minimum(c, 0.0) * (x1 - x0), out)  # h > 0 everywhere
+    out = np.where(c >= H, full, out)
+    return out
+
+
+def _measure_below(c, x0, x1, H, a, xc):
+    """Length of {x in [x0,x1] : h(x) < c}."""
+    c = np.asarray(c, dtype=np.float64)
+    L = x1 - x0
+    inner = (c > 0.0) & (c <= H)
+    s = np.where(inner, a * np.sqrt(np.maximum(H / np.where(c > 0, c, 1.0) - 1.0, 0.0)), 0.0)
+    lo = np.maximum(x0, xc - s)
+    hi = np.minimum(x1, xc + s)
+    above = np.where(inner, np.maximum(hi - lo, 0.0), 0.0)
+    out = L - above
+    out = np.where(c <= 0.0, 0.0, out)
+    out = np.where(c > H, L, out)
+    return out
+
+
+@dataclasses.dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+def cut_geometry(g: Grid, H: float, a: float, xc: float, vf_min: float = 1e-3):
+    """Exact cut-cell face areas and volumes above the ridge.
+
+    Returns FU (nz,ny,nx+1), FV (nz,ny+1,nx), FW (nz+1,ny,nx), VolC (nz,ny,nx).
+    Volume fractions in (0, vf_min) are raised to vf_min; fractions below 1e-9
+    are treated as fully buried (V = 0, dead cell).
+    """
+    nx, ny, nz = g.nx, g.ny, g.nz
+    X = np.arange(nx + 1, dtype=np.float64) * g.dx
+    Z = np.arange(nz + 1, dtype=np.float64) * g.dz
+    if H <= 0.0:
+        FU = np.full((nz, ny, nx + 1), g.dy * g.dz)
+        FV = np.full((nz, ny + 1, nx), g.dx * g.dz)
+        FW = np.full((nz + 1, ny, nx), g.dx * g.dy)
+        V = np.full((nz, ny, nx), g.dx * g.dy * g.dz)
+        return FU, FV, FW, V
+    x0 = X[:-1][None, :]
+    x1 = X[1:][None, :]
+    zlo = Z[:-1][:, None]
+    zhi = Z[1:][:, None]
+    # Open x-z area of cell (k,i): integral of (zhi - clamp(h, zlo, zhi)).
+    Mhi = _int_min_h(np.broadcast_to(zhi, (nz, nx)), np.broadcast_to(x0, (nz, nx)),
+                     np.broadcast_to(x1, (nz, nx)), H, a, xc)
+    Mlo = _int_min_h(np.broadcast_to(zlo, (nz, nx)), np.broadcast_to(x0, (nz, nx)),
+                     np.broadcast_to(x1, (nz, nx)), H, a, xc)
+    clamp_int = Mhi - Mlo + zlo * (x1 - x0)
+    area_xz = zhi * (x1 - x0) - clamp_int
+    area_xz = np.clip(area_xz, 0.0, g.dx * g.dz)
+    vf = area_xz / (g.dx * g.dz)
+    vf = np.where(vf < 1e-9, 0.0, vf)
+    vf_c = np.where((vf > 0.0) & (vf < vf_min), vf_min, vf)
+    V2 = vf_c * g.dx * g.dz * g.dy  # (nz, nx)
+    # x-faces: open vertical length at X_i.
+    hX = H / (1.0 + ((X - xc) / a) ** 2)
+    openz = zhi - np.clip(hX[None, :], zlo, zhi)  # (nz, nx+1)
+    openz = np.where(openz < 1e-9 * g.dz, 0.0, openz)
+    FU2 = openz * g.dy
+    # y-faces: the x-z open area of the cell (terrain independent of y).
+    FV2 = np.where(vf > 0.0, area_xz, 0.0) * 1.0
+    # z-faces: open length of {h < Z_k} over the column's x-range.
+    meas = _measure_below(np.broadcast_to(Z[:, None], (nz + 1, nx)),
+                          np.broadcast_to(x0, (nz + 1, nx)),
+                          np.broadcast_to(x1, (nz + 1, nx)), H, a, xc)
+    meas = np.where(meas < 1e-9 * g.dx, 0.0, meas)
+    FW2 = meas * g.dy
+    FU = np.broadcast_to(FU2[:, None, :], (nz, ny, nx + 1)).copy()
+    FV = np.broadcast_to(FV2[:, None, :], (nz, ny + 1, nx)).copy()
+    FW = np.broadcast_to(FW2[:, None, :], (nz + 1, ny, nx)).copy()
+    V = np.broadcast_to(V2[:, None, :], (nz, ny, nx)).copy()
+    return FU, FV, FW, V
+
+
+def background_state(g: Grid, bubble_amp: float, bubble_xc: float, bubble_yc: float,
+                     bubble_zc: float = 2000.0, bubble_r: float = 2000.0, theta0: float = 300.0):
+    """Hydrostatic dry atmosphere with a Gaussian warm bubble.
+
+    pi(z) = 1 - g z/(Cpd theta0); p = p0 pi^(Cpd/Rd); theta = theta0 + amp exp(-(r/R)^2);
+    T = theta*pi; rho = p/(Rd T); rhotheta = rho*theta.  Cell-centre coordinates.
+    Returns rho, rhotheta, T, each (nz, ny, nx).
+    """
+    xcen = (np.arange(g.nx) + 0.5) * g.dx
+    ycen = (np.arange(g.ny) + 0.5) * g.dy
+    zcen = (np.arange(g.nz) + 0.5) * g.dz
+    Z, Y, X = np.meshgrid(zcen, ycen, xcen, indexing="ij")
+    pi = 1.0 - GRAV * Z / (CPD * theta0)
+    p = P0 * pi ** (CPD / RD)
+    r2 = ((X - bubble_xc) ** 2 + (Y - bubble_yc) ** 2 + (Z - bubble_zc) ** 2) / bubble_r ** 2
+    theta = theta0 + bubble_amp * np.exp(-r2)
+    T = theta * pi
+    rho = p / (RD * T)
+    rhotheta = rho * theta
+    return rho, rhotheta, T
+
+
+@dataclasses.dataclass
+class SoundProblem:
+    """One stage system as handed to SolveSound: geometry, state, constants, b."""
+    grid: Grid
+    np1: int
+    grad_full: int
+    beta0_dt: float
+    grav: float
+    FU: np.ndarray
+    FV: np.ndarray
+    FW: np.ndarray
+    VolC: np.ndarray
+    dxv: np.ndarray
+    dyv: np.ndarray
+    dzv: np.ndarray
+    rho: np.ndarray
+    rhotheta: np.ndarray
+    T: np.ndarray
+    b: np.ndarray
+    name: str = ""
+
+    @property
+    def n(self) -> int:
+        return self.grid.ncell * self.np1
+
+    def active(self) -> np.ndarray:
+        return self.VolC > 0.0
+
+
+def make_problem(nx, ny, nz, dx, dy, dz, *, H=0.0, a=5000.0, ridge_xc=None,
+                 bubble_amp=2.0, bubble_xc=None, bubble_yc=None, dt=2.0,
+                 np1=1, grad_full=1, seed=1, name="") -> SoundProblem:
+    g = Grid(nx, ny, nz, dx, dy, dz)
+    if ridge_xc is None:
+        ridge_xc = 0.5 * nx * dx
+    if bubble_xc is None:
+        bubble_xc = 0.5 * nx * dx
+    if bubble_yc is None:
+        bubble_yc = 0.5 * ny * dy
+    FU, FV, FW, V = cut_geometry(g, H, a, ridge_xc)
+    rho, rhotheta, T = background_state(g, bubble_amp, bubble_xc, bubble_yc)
+    n = g.ncell * np1
+    b = normal(n, seed)
+    act = (V > 0.0).reshape(-1)
+    if np1 == 1:
+        b = np.where(act, b, 0.0)
+    else:
+        b = b.reshape(-1, 2)
+        b[~act, :] = 0.0
+        b = b.reshape(-1)
+    beta0_dt = GAMMA_ROS * dt
+    grav = GRAV if np1 == 2 else 0.0
+    return SoundProblem(g, np1, grad_full, beta0_dt, grav, FU, FV, FW, V,
+                        np.full(nx, dx), np.full(ny, dy), np.full(nz, dz),
+                        rho, rhotheta, T, b, name)
+
+
+def config(idx: int, np1: int = 1, seed: int | None = None, scale: float = 1.0,
+           grad_full: int = 1) -> SoundProblem:
+    """The named workloads.
+
+    1: 64x64x40 box, dx=dy=1 km, dz=250 m, no terrain.
+    2: 256x256x64, dx=dy=500 m, dz=200 m, ridge H=1 km, a=5 km.
+    3: 512x512x96, same spacing, ridge H=2 km.
+    4: the config-1 grid with a ridge H=1 km, a=5 km (adjoint case).
+    ``scale`` shrinks nx, ny (for quick parity runs) keeping nz.
+    """
+    s = seed if seed is not None else idx
+    def sc(n):
+        return max(4, int(round(n * scale)))
+    if idx == 1:
+        return make_problem(sc(64), sc(64), 40, 1000.0, 1000.0, 250.0, H=0.0,
+                            np1=np1, grad_full=grad_full, seed=s, name="cfg1")
+    if idx == 2:
+        return make_problem(sc(256), sc(256), 64, 500.0, 500.0, 200.0, H=1000.0, a=5000.0,
+                            np1=np1, grad_full=grad_full, seed=s, name="cfg2")
+    if idx == 3:
+        return make_problem(sc(512), sc(512), 96, 500.0, 500.0, 200.0, H=2000.0, a=5000.0,
+                            np1=np1, grad_full=grad_full, seed=s, name="cfg3")
+    if idx == 4:
+        return make_problem(sc(64), sc(64), 40, 1000.0, 1000.0, 250.0, H=1000.0, a=5000.0,
+                            np1=np1, grad_full=grad_full, seed=s, name="cfg4")
+    raise ValueError(idx)
+
+
+def ensemble_members(n_members: int = 64, seed: int = 5, np1: int = 1, size_scale: float = 1.0):
+    """Uneven ensemble: nx,ny ~ U{64..256}, nz ~ U{40..64}, bubble ~ U(1,3) K,
+    ridge height ~ U(0, 1.5) km, dx=dy=500 m, dz=200 m; member m uses seed+1+m for b."""
+    u = uniform01(5 * n_members, seed)
+    out = []
+    for m in range(n_members):
+        r = u[5 * m:5 * m + 5]
+        nx = 64 + int(math.floor(r[0] * 193)) if r[0] < 1.0 else 256
+        ny = 64 + int(math.floor(r[1] * 193)) if r[1] < 1.0 else 256
+        nz = 40 + int(math.floor(r[2] * 25)) if r[2] < 1.0 else 64
+        nx = max(8, int(round(nx * size_scale)))
+        ny = max(8, int(round(ny * size_scale)))
+        amp = 1.0 + 2.0 * r[3]
+        H = 1500.0 * r[4]
+        out.append(make_problem(nx, ny, nz, 500.0, 500.0, 200.0, H=H, a=5000.0,
+                                bubble_amp=amp, np1=np1, seed=seed + 1 + m,
+                                name=f"member{m}"))
+    return out

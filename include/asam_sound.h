@@ -1,0 +1,179 @@
+/*
+ * asam_sound.h -- C-ABI of the MI355X (gfx950) sound stage solver.
+ *
+ * Drop-in for ASAM's implicit sound/buoyancy stage solve: the system built by
+ * JacAccGrav (reference SRC/MULTI/JacAccGrav_Mod.f90:304-471) and solved by
+ * SolveSound (reference SRC/SOLU/Iter_Mod.f90:9-30), with the reference's
+ * conventions kept at this boundary:
+ *   - the Krylov operator is MatVec1 = -A x (reference SRC/MATRIX/MatVec_Mod.f90:232),
+ *     so SolveSound solves (-A) x = b;
+ *   - the preconditioner has PSolve1 semantics, -M^-1 x (reference
+ *     SRC/MATRIX/PreCond_Mod.f90:1555-1571), M the vertical-line block of A;
+ *   - vectors are the `p` component of TYPE(PressureVelocity)
+ *     (reference SRC/MODULE/VecPrVel_Mod.f90:11-18): p(1:np1, nx, ny, nz),
+ *     component fastest, then x, y, z (Fortran column-major);
+ *   - geometry and state arrays use the reference's interior/face shapes:
+ *     FU(0:nx,1:ny,1:nz), FV(1:nx,0:ny,1:nz), FW(1:nx,1:ny,0:nz), cells (nx,ny,nz).
+ *
+ * All functions return 0 on success, a negative ASAM_E_* code on error; they
+ * never abort the host.  A context is bound to the HIP device that was
+ * current when it was created and owns one HIP stream; calls on one context
+ * are not re-entrant, calls on different contexts are independent.
+ */
+#ifndef ASAM_SOUND_H
+#define ASAM_SOUND_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASAM_OK 0
+#define ASAM_E_INVALID (-1)   /* bad argument / size */
+#define ASAM_E_HIP (-2)       /* HIP runtime error */
+#define ASAM_E_STATE (-3)     /* call order (e.g. solve before setup) */
+#define ASAM_E_NOMEM (-4)     /* device allocation failed */
+#define ASAM_E_NAN (-5)       /* non-finite input or iterate */
+
+/* Krylov method (MethSound, reference SRC/MODULE/Control_Mod.f90:64). */
+#define ASAM_METHOD_BICGSTAB 0 /* SpMatrix4CellPar_Mod.f90:245-348 sequence */
+#define ASAM_METHOD_FGMRES 1   /* Iter_Mod.f90:121-199 (reference default) */
+#define ASAM_METHOD_GMRES 2    /* Iter_Mod.f90:32-119 (left preconditioned) */
+
+/* Arithmetic of the solve. */
+#define ASAM_PREC_FP64 0  /* plain FP64 drop-in */
+#define ASAM_PREC_MIXED 1 /* FP32 operator/preconditioner/inner Krylov; FP64 residual,
+                             dots and refinement */
+
+/* Solve status (asam_sound_info.status). */
+#define ASAM_CONVERGED 0
+#define ASAM_NOT_CONVERGED 1
+#define ASAM_BREAKDOWN 2 /* rho == 0 in BiCGStab (reference returns Tol = 0) */
+
+typedef struct asam_sound_ctx asam_sound_ctx;
+
+/* Grid and physical constants of one block (one stage system). */
+typedef struct {
+  int nx, ny, nz;    /* interior cells */
+  int np1;           /* 1: MultiTriT (rho*theta'), 2: MultiTriTB (rho*theta', rho') */
+  int grad_full;     /* Control_Mod GradFull (reference default 1) */
+  double beta0_dt;   /* beta0*dtP: Rosenbrock gamma times the step */
+  double grav;       /* GravComp: Grav for MultiTriTB, 0 otherwise */
+  double fac_anela;  /* Physics_Mod FacAnela: 1, 0 when anelastic */
+} asam_sound_grid;
+
+/* Solver controls (reference &ModelControl knobs). */
+typedef struct {
+  int method;        /* ASAM_METHOD_* */
+  int step_gmres;    /* restart length m (StepGMRES, default 10), <= 32 */
+  int precision;     /* ASAM_PREC_* */
+  double inner_tol;  /* mixed: relative reduction asked of each FP32 inner solve */
+  int max_inner;     /* mixed: cap on inner iterations per refinement step */
+  int max_outer;     /* mixed: cap on refinement steps */
+} asam_sound_control;
+
+/* Outcome of one solve. */
+typedef struct {
+  int iterations;         /* BiCGStab: completed iterations (a half-step exit counts as one);
+                             (F)GMRES: Arnoldi steps summed over restarts;
+                             mixed: inner iterations summed over refinement steps */
+  int restarts;           /* (F)GMRES restart cycles / mixed refinement steps */
+  int status;             /* ASAM_CONVERGED / ASAM_NOT_CONVERGED / ASAM_BREAKDOWN */
+  int reserved;
+  double rel_residual;    /* final ||r|| / ||r0|| as tracked by the method */
+  double abs_residual;    /* final ||r|| as tracked by the method */
+  double ms;              /* device time of the solve (HIP events), milliseconds */
+} asam_sound_info;
+
+/* ---- context ---------------------------------------------------------- */
+int asam_sound_create(asam_sound_ctx **ctx, const asam_sound_grid *grid);
+int asam_sound_destroy(asam_sound_ctx *ctx);
+int asam_sound_default_control(asam_sound_control *ctl);
+int asam_sound_set_control(asam_sound_ctx *ctx, const asam_sound_control *ctl);
+/* Last error message of this context (static storage of the context). */
+const char *asam_sound_last_error(const asam_sound_ctx *ctx);
+
+/* ---- operator set-up (once per time step; reference JacAccGrav) -------- */
+/* From geometry and dry thermodynamic state, host arrays:
+ *   FU (nx+1)*ny*nz, FV nx*(ny+1)*nz, FW nx*ny*(nz+1), VolC/rho/rhotheta/T nx*ny*nz,
+ *   dx nx, dy ny, dz nz.
+ * Replaces JacAccGrav -> WeightDiagCompute('Density') -> LaplTCompute/LaplTBCompute
+ * (JacAccGrav_Mod.f90:304-471, 4254-4274, 893-1484). */
+int asam_sound_setup(asam_sound_ctx *ctx, const double *FU, const double *FV, const double *FW,
+                     const double *VolC, const double *dx, const double *dy, const double *dz,
+                     const double *rho, const double *rhotheta, const double *T);
+/* From geometry and precomputed weights DTU/DTV/DTW (face theta, face shapes) and DUT
+ * (cells) -- the inputs of LaplTCompute (JacAccGrav_Mod.f90:893-1055). */
+int asam_sound_setup_weights(asam_sound_ctx *ctx, const double *FU, const double *FV,
+                             const double *FW, const double *VolC, const double *dx,
+                             const double *dy, const double *dz, const double *DTU,
+                             const double *DTV, const double *DTW, const double *DUT);
+
+/* ---- the drop-in solve (reference SolveSound(x,b,MaxIter,Tol)) --------- */
+/* Host buffers of np1*nx*ny*nz doubles.  x: initial guess in, solution out.
+ * maxiter: BiCGStab iteration cap / (F)GMRES restart cycles (QMRMaxIter), in; on
+ * return for BiCGStab the reference's convention (-Iter half-step exit, Iter full-step
+ * exit, 2*Iter not converged, Iter with tol=0 on breakdown), unchanged otherwise.
+ * tol: relative tolerance in; for BiCGStab the final ||r|| out (reference semantics).
+ * info may be NULL. */
+int asam_sound_solve(asam_sound_ctx *ctx, double *x, const double *b, int *maxiter,
+                     double *tol, asam_sound_info *info);
+/* Same on device buffers (allocated by asam_dev_alloc or any HIP allocation on the
+ * context's device); ordered on the context's stream, synchronous on return. */
+int asam_sound_solve_device(asam_sound_ctx *ctx, double *x, const double *b, int *maxiter,
+                            double *tol, asam_sound_info *info);
+
+/* ---- operator pieces (Krylov plug points) ------------------------------ */
+/* y = MatVec1(x) = -A x (MatVec_Mod.f90:8-18,186-232); host buffers. */
+int asam_sound_apply(asam_sound_ctx *ctx, double *y, const double *x);
+/* y = -A^T x (adjoint operator; no reference counterpart). */
+int asam_sound_apply_T(asam_sound_ctx *ctx, double *y, const double *x);
+/* y = PSolve1(x) = -M^-1 x, M = vertical-line block of A (PreCond_Mod.f90:1555-1571). */
+int asam_sound_psolve(asam_sound_ctx *ctx, double *y, const double *x);
+/* y = -M^-T x. */
+int asam_sound_psolve_T(asam_sound_ctx *ctx, double *y, const double *x);
+/* Device-buffer forms of the four. */
+int asam_sound_apply_device(asam_sound_ctx *ctx, double *y, const double *x, int transpose);
+int asam_sound_psolve_device(asam_sound_ctx *ctx, double *y, const double *x, int transpose);
+
+/* ---- adjoint solve (backward half of the forward/backward pair) -------- */
+/* Solves (-A)^T lambda = g with the transposed line preconditioner, same method and
+ * semantics as asam_sound_solve. */
+int asam_sound_solve_T(asam_sound_ctx *ctx, double *lambda, const double *g, int *maxiter,
+                       double *tol, asam_sound_info *info);
+int asam_sound_solve_T_device(asam_sound_ctx *ctx, double *lambda, const double *g,
+                              int *maxiter, double *tol, asam_sound_info *info);
+
+/* Sensitivity of s = mu^T (A x) to the cell thetas (theta = rhotheta/rho) that enter A
+ * through the volume-weighted face theta (CellToFaceRhoWeight, JacAccGrav_Mod.f90:3316):
+ * grad[c] += d(mu^T A x)/d theta_c, device buffers (grad nx*ny*nz). */
+int asam_sound_theta_vjp_device(asam_sound_ctx *ctx, double *grad, const double *mu,
+                                const double *x);
+
+/* ---- device memory helpers (on the context's device) ------------------- */
+int asam_dev_alloc(asam_sound_ctx *ctx, void **ptr, size_t bytes);
+int asam_dev_free(asam_sound_ctx *ctx, void *ptr);
+int asam_dev_upload(asam_sound_ctx *ctx, void *dst, const void *src, size_t bytes);
+int asam_dev_download(asam_sound_ctx *ctx, void *dst, const void *src, size_t bytes);
+int asam_dev_memset(asam_sound_ctx *ctx, void *dst, int value, size_t bytes);
+/* Device-side linear algebra used by Rosenbrock drivers (all on the context's stream):
+ * y = a*x + b*y over n doubles; dot = sum(x*y) (FP64, deterministic order). */
+int asam_dev_axpby(asam_sound_ctx *ctx, size_t n, double a, const double *x, double b,
+                   double *y);
+int asam_dev_dot(asam_sound_ctx *ctx, size_t n, const double *x, const double *y, double *dot);
+/* y = V .* x (component-wise cell volume, the mass matrix of the stage system). */
+int asam_dev_mass(asam_sound_ctx *ctx, double *y, const double *x);
+
+/* ---- measurement hooks -------------------------------------------------- */
+/* Average device time (ms, HIP events on the context's stream) of one launch of the
+ * stencil apply (which=0), line preconditioner (which=1) or fused BiCGStab update
+ * (which=2), in FP64 (fp32=0) or FP32 (fp32=1), over reps launches. */
+int asam_sound_time_kernel(asam_sound_ctx *ctx, int which, int fp32, int reps, double *ms);
+/* Kernel launch counts of the last solve, per kernel class (apply, line, blas, reduce). */
+int asam_sound_last_counts(const asam_sound_ctx *ctx, int counts[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,78 @@
+/*
+ * oracle.h -- CPU restatement (FP64) of ASAM's sound stage solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
+ * path and the CPU baseline timed beside it.  Nothing in the product
+ * (asam_amd/, include/) links or calls it.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the reference's SRC/ directory).
+ */
+#ifndef ASAM_ORACLE_H
+#define ASAM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int nx, ny, nz;   /* interior cells */
+  int np1;          /* 1: MultiTriT (rho*theta), 2: MultiTriTB (rho*theta, rho) */
+  int grad_full;    /* Control_Mod GradFull */
+  double beta0_dt;  /* beta0*dtP */
+  double grav;      /* GravComp (TB only) */
+  double fac_anela; /* Physics_Mod FacAnela */
+} orc_grid;
+
+typedef struct orc_op orc_op;
+
+/* Assemble the Neumann sound matrix from geometry and state
+ * (JacAccGrav -> WeightDiagCompute 'Density' -> LaplTCompute / LaplTBCompute). */
+orc_op *orc_assemble(const orc_grid *g, const double *FU, const double *FV, const double *FW,
+                     const double *VolC, const double *dx, const double *dy, const double *dz,
+                     const double *rho, const double *rhotheta, const double *T);
+/* Same, but with the thermodynamic weights given directly (LaplTCompute inputs). */
+orc_op *orc_assemble_weights(const orc_grid *g, const double *FU, const double *FV,
+                             const double *FW, const double *VolC, const double *dx,
+                             const double *dy, const double *dz, const double *DTU,
+                             const double *DTV, const double *DTW, const double *DUT);
+void orc_free(orc_op *A);
+long orc_n(const orc_op *A);
+
+/* Face weights computed by WeightDiagCompute ('Density'): theta at faces and DUT. */
+void orc_weights(const orc_grid *g, const double *VolC, const double *rho,
+                 const double *rhotheta, const double *T, double *DTU, double *DTV,
+                 double *DTW, double *DUT);
+
+/* Export the nested tri-diagonal coefficients (x-fastest, face (c,c+e) stored at c).
+ * T: d[N],ux,lx,uy,ly,uz,lz[N].  TB: d[4N] (2x2 column-major), uz,lz[4N], ux,lx,uy,ly[2N]. */
+void orc_export(const orc_op *A, double *d, double *ux, double *lx, double *uy, double *ly,
+                double *uz, double *lz);
+
+void orc_apply(const orc_op *A, double *y, const double *x);     /* y = A x      (MatVecT_3D) */
+void orc_matvec1(const orc_op *A, double *y, const double *x);   /* y = -A x     (MatVec1)    */
+void orc_matvec1_T(const orc_op *A, double *y, const double *x); /* y = -A^T x              */
+void orc_psolve1(const orc_op *A, double *y, const double *x);   /* y = -M^-1 x  (PSolve1)    */
+void orc_psolve1_T(const orc_op *A, double *y, const double *x); /* y = -M^-T x             */
+void orc_line_solve(const orc_op *A, double *y, const double *x); /* y = M^-1 x (Solve_1D)  */
+
+/* Krylov solvers of the operator -A with preconditioner -M^-1.  transpose!=0 runs the
+ * same sequence with -A^T and -M^-T (adjoint solve). */
+/* BiCGStab (SpMatrix4CellPar_Mod.f90:245-348): maxiter/tol in-out with reference semantics. */
+int orc_bicgstab(const orc_op *A, double *x, const double *b, int *maxiter, double *tol,
+                 int transpose);
+/* FGMRES(m) (Iter_Mod.f90:121-199): maxiter = restart cycles. Returns total Arnoldi steps. */
+int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
+               int transpose, double *rel_res);
+/* GMRES(m), left preconditioned (Iter_Mod.f90:32-119). Returns total Arnoldi steps. */
+int orc_gmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
+              int transpose, double *rel_res);
+
+double orc_dot(long n, const double *x, const double *y);
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
