@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libasam_sound.so")
-SOURCES = ["stencil.hip", "blas.hip", "solver.hip", "batch.hip"]
+SOURCES = ["setup.hip", "apply.hip", "line.hip", "blas.hip", "solver.hip", "batch.hip"]
 
 
 def hipcc() -> str:

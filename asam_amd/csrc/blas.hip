@@ -7,6 +7,7 @@
 // dot product is bitwise reproducible run to run.
 #include <math.h>
 
+#include "fin.h"
 #include "internal.h"
 
 namespace asam {
@@ -242,142 +243,49 @@ void launch_mass(hipStream_t s, const Geom& g, const double* vol, double* y, con
 }
 
 // ------------------------------------------------------------- finalize
-__global__ __launch_bounds__(256) void k_finalize(int kind, int nq, int nblk,
-                                                  const double* partial, KState* st, int k,
-                                                  int i, double* out) {
-  __shared__ double sh[4][256];
+// One workgroup of 1024 threads: every thread sums a fixed strided subset of the
+// partials, then a fixed wavefront butterfly and a fixed 16-wave sum -> identical
+// results run to run.  Thread 0 then runs the scalar step.
+__global__ __launch_bounds__(1024) void k_finalize(int kind, int nq, int nblk, int nblk_hi,
+                                                   const double* partial, KState* st, int k,
+                                                   int i, double* out) {
+  __shared__ double sh[4][16];
   const int tid = threadIdx.x;
-  for (int q = 0; q < nq; ++q) {
-    double s = 0.0;
-    for (int b = tid; b < nblk; b += 256) s += partial[q * kRedBlocks + b];
-    sh[q][tid] = s;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (q < nq) {
+      const int nb = q < 2 ? nblk : nblk_hi;
+      const double* pq = partial + (long)q * kRedBlocks;
+      double s = 0.0;
+      for (int b = tid; b < nb; b += 1024) s += pq[b];
+      v[q] = s;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sh[q][tid >> 6] = v[q];
   }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w)
-      for (int q = 0; q < nq; ++q) sh[q][tid] += sh[q][tid + w];
-    __syncthreads();
-  }
   if (tid != 0) return;
   double S[4];
-  for (int q = 0; q < 4; ++q) S[q] = q < nq ? sh[q][0] : 0.0;
-  switch (kind) {
-    case FIN_SUMS:
-      for (int q = 0; q < 4; ++q) st->sum[q] = S[q];
-      break;
-    case FIN_DOT:
-      out[0] = S[0];
-      break;
-    case FIN_BICG_INIT: {
-      // rNorm0 = ||r||; iteration 1 computes rho = rTilde.r = ||r||^2
-      st->rnorm0 = sqrt(S[0]);
-      st->rnorm = st->rnorm0;
-      st->rho = S[0];
-      st->iter = 1;
-      st->done = 0; st->half = 0; st->pad = 0;
-      st->status = 1;
-      if (st->rho == 0.0) {  // reference: MaxIter = Iter, Tol = 0
-        st->result = 1; st->rnorm = 0.0; st->status = 2; st->done = 1; st->pad = 1;
-      }
-      break;
-    }
-    case FIN_BICG_ALPHA:
-      if (st->done) break;
-      st->alpha = st->rho / S[1];
-      break;
-    case FIN_BICG_HALF: {
-      if (st->done) break;
-      double rn = sqrt(S[0]);
-      st->rnorm = rn;
-      if (rn <= st->tol * st->rnorm0) {
-        st->half = 1; st->pad = 1; st->result = -st->iter; st->status = 0;
-      }
-      break;
-    }
-    case FIN_BICG_OMEGA:
-      if (st->done || st->pad) break;
-      st->omega = S[1] / S[0];
-      break;
-    case FIN_BICG_END: {
-      if (st->done) break;
-      if (st->half) { st->done = 1; st->pad = 1; break; }
-      double rn = sqrt(S[0]);
-      st->rnorm = rn;
-      if (rn <= st->tol * st->rnorm0) {
-        st->result = st->iter; st->status = 0; st->done = 1;
-      } else if (st->iter == st->maxiter) {
-        st->result = 2 * st->iter; st->status = 1; st->done = 1;
-      } else {
-        st->rho_old = st->rho;
-        st->rho = S[1];
-        if (st->rho == 0.0) {
-          st->result = st->iter + 1; st->rnorm = 0.0; st->status = 2; st->done = 1;
-        } else {
-          st->beta = (st->rho / st->rho_old) * (st->alpha / st->omega);
-          st->iter += 1;
-        }
-      }
-      st->pad = st->done;
-      break;
-    }
-    case FIN_GM_INIT: {
-      // start of a restart cycle: beta = ||b - Op x||
-      if (st->done) { st->pad = 1; st->cycle_done = 1; break; }
-      double beta = sqrt(S[0]);
-      if (st->cycles == 0) st->grho0 = beta;
-      st->gbeta = beta;
-      if (beta <= 0.0) {
-        st->done = 1; st->pad = 1; st->cycle_done = 1; st->grho = 0.0; st->gstatus = 0;
-        break;
-      }
-      st->vscale[0] = 1.0 / beta;
-      st->bhat[0] = beta * 1.0;
-      st->nR = st->m;
-      st->cycle_done = 0;
-      st->pad = 0;
-      break;
-    }
-    case FIN_GM_H:
-      if (st->cycle_done) break;
-      st->H[k * kMaxM + i] = (nq == 2) ? S[1] * st->vscale[0] : S[0];
-      break;
-    case FIN_GM_LAST: {
-      if (st->cycle_done) break;
-      double* H = st->H;
-      const int M = kMaxM;
-      H[(i + 1) * M + i] = sqrt(S[0]);
-      st->vscale[i + 1] = 1.0 / H[(i + 1) * M + i];
-      for (int q = 1; q <= i; ++q) {
-        double temp = st->cs[q - 1] * H[(q - 1) * M + i] + st->sn[q - 1] * H[q * M + i];
-        H[q * M + i] = -st->sn[q - 1] * H[(q - 1) * M + i] + st->cs[q - 1] * H[q * M + i];
-        H[(q - 1) * M + i] = temp;
-      }
-      double delta = sqrt(H[i * M + i] * H[i * M + i] + H[(i + 1) * M + i] * H[(i + 1) * M + i]);
-      st->cs[i] = H[i * M + i] / delta;
-      st->sn[i] = H[(i + 1) * M + i] / delta;
-      H[i * M + i] = st->cs[i] * H[i * M + i] + st->sn[i] * H[(i + 1) * M + i];
-      st->bhat[i + 1] = -st->sn[i] * st->bhat[i];
-      st->bhat[i] = st->cs[i] * st->bhat[i];
-      st->grho = fabs(st->bhat[i + 1]);
-      st->total += 1;
-      if (st->grho <= st->gtol * st->grho0) {
-        st->nR = i + 1;
-        st->cycle_done = 1;
-      } else {
-        st->nR = st->m;
-        if (i == st->m - 1) st->cycle_done = 1;
-      }
-      break;
-    }
-    default:
-      break;
+  for (int q = 0; q < 4; ++q) {
+    double s = 0.0;
+    for (int w = 0; w < 16; ++w) s += sh[q][w];
+    S[q] = q < nq ? s : 0.0;
   }
+  fin_logic(kind, S, st, k, i, nq, out);
 }
 
 void launch_finalize(hipStream_t s, int kind, int nq, int nblk, const double* partial,
-                     KState* st, int k, int i, double* out) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, kind, nq, nblk, partial, st, k, i,
-                     out);
+                     KState* st, int k, int i, double* out, int nblk_hi) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, kind, nq, nblk,
+                     nblk_hi > 0 ? nblk_hi : nblk, partial, st, k, i, out);
 }
 
 // (F)GMRES back-substitution (Iter_Mod.f90:181-188) and end-of-cycle decision.
@@ -403,49 +311,6 @@ __global__ void k_gm_backsolve(KState* st) {
 
 void launch_gm_backsolve(hipStream_t s, KState* st) {
   hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, st);
-}
-
-// Mixed refinement: residual norm of the FP64 outer residual in sum (partial q0).
-__global__ __launch_bounds__(256) void k_mixed_outer(int nblk, const double* partial,
-                                                     KState* st) {
-  __shared__ double sh[256];
-  const int tid = threadIdx.x;
-  double s = 0.0;
-  for (int b = tid; b < nblk; b += 256) s += partial[b];
-  sh[tid] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (tid < w) sh[tid] += sh[tid + w];
-    __syncthreads();
-  }
-  if (tid != 0) return;
-  if (st->outer_done) return;
-  double r = sqrt(sh[0]);
-  if (st->outer == 0) st->outer_r0 = r;
-  st->outer_r = r;
-  if (r <= st->outer_tol * st->outer_r0 || r == 0.0) {
-    st->outer_done = 1;
-    st->gstatus = 0;
-    st->done = 1; st->pad = 1;
-    return;
-  }
-  // Inner BiCGStab on A d = r (d = 0): rNorm0 = ||r||, rho = ||r||^2 (rTilde = r).
-  // Ask for the inner inner_tol reduction, but not more than the outer target needs.
-  double need = 0.5 * st->outer_tol * st->outer_r0 / r;
-  double itol = st->inner_tol;
-  if (need > itol) itol = need;
-  if (itol > 0.5) itol = 0.5;
-  st->tol = itol;
-  st->rnorm0 = r;
-  st->rnorm = r;
-  st->rho = sh[0];
-  st->iter = 1;
-  st->done = 0; st->half = 0; st->pad = 0; st->status = 1;
-  st->outer += 1;
-}
-
-void launch_mixed_outer(hipStream_t s, int nblk, const double* partial, KState* st) {
-  hipLaunchKernelGGL(k_mixed_outer, dim3(1), dim3(256), 0, s, nblk, partial, st);
 }
 
 }  // namespace asam

@@ -30,11 +30,36 @@ struct Coef {
   T *pl = nullptr, *pt = nullptr, *pw = nullptr;
 };
 
+// Launch-shape knobs (set by the autotuner or asam_sound_set_tuning).
+struct Tuning {
+  int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread
+  int apply_wg = 4096;    // target workgroups per apply launch
+  int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
+  int line_lds = 1;       // stage forward-sweep results in LDS (else in the output vector)
+  int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
+  int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
+  int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
+};
+
 struct Geom {
   int nx, ny, nz, np1;
   long N;      // cells
   double fa;   // FacAnela
+  Tuning tune;
 };
+
+// Index of the line-factor entry of cell (i,j,k).  Column-tiled layout: 64 consecutive
+// x-columns of one y-row form a tile whose nz levels are contiguous, so a wavefront
+// (one tile) streams one contiguous block per array through its sweeps.
+__host__ __device__ inline long fac_index(const Geom& g, int i, int j, int k) {
+  if (g.tune.line_tiled) {
+    const long gxl = (g.nx + 63) / 64;
+    return (((long)j * gxl + (i >> 6)) * g.nz + k) * 64 + (i & 63);
+  }
+  return i + (long)g.nx * (j + (long)g.ny * k);
+}
+// Entries per line-factor array (tiled layout pads x to a multiple of 64).
+inline long fac_cells(const Geom& g) { return ((long)(g.nx + 63) / 64) * 64 * g.ny * g.nz; }
 
 // Device-resident scalar state of one Krylov solve (written only by the
 // single-thread tails of the finalize kernels, read by every kernel).
@@ -110,6 +135,7 @@ struct LineReq {
   double* partial = nullptr;
   const int* done = nullptr;
   int use_lds = 1;
+  int qslot = 0;  // partial-sum quantity slot of the LINE_BICG_S norm
 };
 int line_blocks(const Geom& g);
 void launch_line(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
@@ -144,17 +170,20 @@ enum FinKind {
   FIN_BICG_ALPHA,       // alpha = rho / (rt.v)
   FIN_BICG_HALF,        // rnorm = |s|; half-step exit test
   FIN_BICG_OMEGA,       // omega = (t.s)/(t.t)
+  FIN_BICG_HALF_OMEGA,  // half-step exit test on |s| (slot 2), else omega (slots 0, 1)
   FIN_BICG_END,         // rnorm = |r|; exit tests; rho, beta for the next iteration
   FIN_GM_INIT,          // beta = |v0|; restart / exit bookkeeping
   FIN_GM_H,             // H[k][i] = sum   (k <= i)
   FIN_GM_LAST,          // H[i+1][i] = |w|, Givens, convergence test
   FIN_DOT,              // store sum[0] to host-visible double (out)
+  FIN_MIXED_OUTER,      // FP64 refinement residual -> next FP32 inner solve or exit
 };
+// Quantities q < 2 were written by nblk workgroups, q >= 2 by nblk_hi workgroups.
 void launch_finalize(hipStream_t s, int kind, int nq, int nblk, const double* partial,
-                     KState* st, int k, int i, double* out);
+                     KState* st, int k, int i, double* out, int nblk_hi = 0);
 // Back-substitution of the (F)GMRES least-squares problem (single thread).
 void launch_gm_backsolve(hipStream_t s, KState* st);
-// Mixed refinement control: after the FP64 residual of the refinement step.
-void launch_mixed_outer(hipStream_t s, int nblk, const double* partial, KState* st);
+
+void launch_copy16(hipStream_t s, long n16, const void* a, void* b);
 
 }  // namespace asam

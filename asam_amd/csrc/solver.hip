@@ -44,9 +44,22 @@ struct asam_sound_ctx {
   double* gtmp = nullptr;
   std::vector<void*> allocs;
   int counts[8] = {0};
-  int last_iters = 4;
+  int last_iters = 4;     // BiCGStab iterations of the last plain solve (first chunk size)
+  int inner_hist[32] = {0};  // inner iterations per refinement step of the last mixed solve
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
+  // live per-kernel timing (asam_sound_profile_enable)
+  struct ProfRec {
+    int cls, tag, iter;
+    double bytes, half_bytes;
+    hipEvent_t e0, e1;
+  };
+  int prof_on = 0;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_next = 0;
+  std::vector<ProfRec> prof_pending;
+  double prof_ms[8] = {0}, prof_bytes[8] = {0};
+  int prof_launches[8] = {0};
 };
 
 #define HIPCHK(ctx, expr)                                                         \
@@ -98,6 +111,51 @@ extern "C" int asam_sound_default_control(asam_sound_control* ctl) {
   return 0;
 }
 
+extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning* t) {
+  if (!c || !t) return ASAM_E_INVALID;
+  t->apply_variant = c->g.tune.apply_variant;
+  t->apply_wg = c->g.tune.apply_wg;
+  t->line_window = c->g.tune.line_window;
+  t->line_lds = c->g.tune.line_lds;
+  t->fused_tail = 0;
+  t->apply_xcd = c->g.tune.apply_xcd;
+  t->apply_bx = c->g.tune.apply_bx;
+  t->line_tiled = c->g.tune.line_tiled;
+  return 0;
+}
+
+extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning* t) {
+  if (!c || !t) return ASAM_E_INVALID;
+  if (t->apply_variant < 0 || t->apply_variant > 1) return fail(c, ASAM_E_INVALID, "apply_variant");
+  if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
+  if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
+    return fail(c, ASAM_E_INVALID, "line_window");
+  Tuning nt;
+  nt.apply_variant = t->apply_variant;
+  nt.apply_wg = t->apply_wg;
+  nt.line_window = t->line_window;
+  nt.line_lds = t->line_lds ? 1 : 0;
+  if (t->fused_tail) return fail(c, ASAM_E_INVALID, "fused_tail is not supported");
+  if (t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
+    return fail(c, ASAM_E_INVALID, "apply_bx");
+  nt.apply_xcd = t->apply_xcd ? 1 : 0;
+  nt.apply_bx = t->apply_bx;
+  nt.line_tiled = t->line_tiled ? 1 : 0;
+  Geom tg = c->g;
+  tg.tune = nt;
+  if (apply_blocks(tg) > kRedBlocks) return fail(c, ASAM_E_INVALID, "apply_wg too large");
+  const bool relayout = nt.line_tiled != c->g.tune.line_tiled;
+  c->g.tune = nt;
+  if (relayout && c->ready) {  // the line factors live in the layout the tuning names
+    HIPCHK(c, hipSetDevice(c->device));
+    launch_factor(c->stream, c->g, c->c64);
+    launch_convert(c->stream, c->g, c->c64, c->c32);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
 extern "C" const char* asam_sound_last_error(const asam_sound_ctx* ctx) {
   return ctx ? ctx->err.c_str() : "null context";
 }
@@ -111,7 +169,9 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
   if (N > (1L << 31)) return ASAM_E_INVALID;
   {
     // every reduction launch must fit the partial-sum slots
-    Geom tg{grid->nx, grid->ny, grid->nz, grid->np1, N, 1.0};
+    Geom tg;
+    tg.nx = grid->nx; tg.ny = grid->ny; tg.nz = grid->nz; tg.np1 = grid->np1; tg.N = N;
+    tg.fa = 1.0;
     if (apply_blocks(tg) > kRedBlocks || line_blocks(tg) > kRedBlocks ||
         (long)((grid->nx + 63) / 64) * ((grid->ny + 3) / 4) > kRedBlocks)
       return ASAM_E_INVALID;
@@ -128,7 +188,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreate(&c->ev0));
   HIPCHK(c, hipEventCreate(&c->ev1));
-  const long P = (long)grid->np1 * grid->np1 * N;
+  const long P = (long)grid->np1 * grid->np1 * fac_cells(c->g);
   auto coef64 = [&](Coef<double>& k) -> int {
     int r = 0;
     r |= dalloc(c, &k.kx, N); r |= dalloc(c, &k.ky, N); r |= dalloc(c, &k.kz, N);
@@ -174,6 +234,7 @@ extern "C" int asam_sound_destroy(asam_sound_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->st_host) (void)hipHostFree(c->st_host);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -280,35 +341,120 @@ extern "C" int asam_sound_setup_weights(asam_sound_ctx* c, const double* FU, con
 // ------------------------------------------------------------------ enqueue helpers
 namespace {
 
+// Algorithmic bytes of one launch: every distinct array element read or written once.
+double apply_bytes(const Geom& g, const ApplyReq& r) {
+  const double N = (double)g.N, NP = g.np1;
+  const double wc = r.prec ? 4 : 8, coefw = g.np1 == 1 ? 5 : 9;
+  const double sx = r.vx ? 4 : 8, so = r.vo ? 4 : 8, su = r.vu ? 4 : 8;
+  double b = N * coefw * wc + N * NP * (sx + so);
+  if (r.dx) b += N * NP * 4;
+  if (r.xw) b += N * NP * 8;
+  if (r.b) b += N * NP * 8;
+  if (r.y2) b += N * NP * so;
+  if (r.dots & DOT_YU) b += N * NP * su;
+  return b;
+}
+double line_bytes(const Geom& g, const LineReq& r, bool first) {
+  const double N = (double)g.N, NP = g.np1;
+  const double wc = r.prec ? 4 : 8, wv = r.vt ? 4 : 8;
+  double vec = r.mode == LINE_PLAIN ? 2 : (r.mode == LINE_BICG_P ? (first ? 3 : 5) : 4);
+  return N * 3 * NP * NP * wc + N * NP * wv * vec;
+}
+
 struct Enq {
   asam_sound_ctx* c;
   hipStream_t s;
   int tr;
+  int iter = 0;  // BiCGStab iteration index of the launches being enqueued (profiling)
   explicit Enq(asam_sound_ctx* cc, int t) : c(cc), s(cc->stream), tr(t) {}
 
+  hipEvent_t ev() {
+    if (c->ev_next >= c->ev_pool.size()) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_next++];
+  }
+  // Brackets one launch with events when profiling; tag 1: the launch is skipped by a
+  // BiCGStab half-step exit, tag 2: it does the half-step x update (half_bytes).
+  template <typename F>
+  void timed(int cls, double bytes, F&& launch, int tag = 0, double half_bytes = 0.0) {
+    if (!c->prof_on) { launch(); return; }
+    hipEvent_t e0 = ev(), e1 = ev();
+    if (!e0 || !e1) { launch(); return; }
+    (void)hipEventRecord(e0, s);
+    launch();
+    (void)hipEventRecord(e1, s);
+    c->prof_pending.push_back({cls, tag, iter, bytes, half_bytes, e0, e1});
+  }
+
+  // Launch a kernel, then (kind >= 0) the finalize of its reduction.
   void apply(int prec, int vx, int vo, int vu, const void* x, void* y, const double* b,
              void* y2, const void* u, int dots, const int* done, const float* dx = nullptr,
-             double* xw = nullptr) {
+             double* xw = nullptr, int tag = 0, int fkind = -1, int fnq = 0) {
     ApplyReq r;
     r.tr = tr; r.prec = prec; r.vx = vx; r.vo = vo; r.vu = vu; r.x = x; r.y = y; r.b = b;
     r.y2 = y2; r.u = u; r.dots = dots; r.partial = c->partial; r.done = done; r.dx = dx;
     r.xw = xw;
-    launch_apply(s, c->g, c->c64, c->c32, r);
+    timed(prec ? 1 : 0, apply_bytes(c->g, r), [&] { launch_apply(s, c->g, c->c64, c->c32, r); },
+          tag);
     c->counts[0]++;
+    if (fkind >= 0) fin(fkind, fnq, apply_blocks(c->g), 0, 0, tag);
   }
   void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
-            const double* scale, const int* done) {
+            const double* scale, const int* done, int qslot = 0) {
     LineReq r;
     r.tr = tr; r.prec = prec; r.vt = prec; r.mode = mode; r.src = src; r.p = p; r.v = v;
     r.out = out; r.scale = scale; r.st = c->st; r.partial = c->partial; r.done = done;
-    launch_line(s, c->g, c->c64, c->c32, r);
+    r.qslot = qslot;
+    timed(2 + prec, line_bytes(c->g, r, iter == 1),
+          [&] { launch_line(s, c->g, c->c64, c->c32, r); });
     c->counts[1]++;
   }
-  void fin(int kind, int nq, int nblk, int k = 0, int i = 0) {
-    launch_finalize(s, kind, nq, nblk, c->partial, c->st, k, i, nullptr);
+  void fin(int kind, int nq, int nblk, int k = 0, int i = 0, int tag = 0, int nblk_hi = 0) {
+    timed(6, 8.0 * nq * nblk,
+          [&] {
+            launch_finalize(s, kind, nq, nblk, c->partial, c->st, k, i, nullptr, nblk_hi);
+          },
+          tag);
     c->counts[3]++;
   }
+  void bicg_update(int vt, void* x, const void* ph, const void* rh, void* r, const void* t,
+                   const void* rt) {
+    const double w = vt ? 4 : 8, nv = (double)c->n;
+    timed(4 + vt, 8 * nv * w,
+          [&] { launch_bicg_update(s, c->n, vt, x, ph, rh, r, t, rt, c->st, c->partial); }, 2,
+          3 * nv * w);
+    c->counts[2]++;
+    fin(FIN_BICG_END, 2, blas_blocks(c->n));
+  }
+  void mgs(int vt, void* w, const void* vprev, const void* vk, int k, int i, int chk = 0) {
+    const double ws = vt ? 4 : 8, nv = (double)c->n;
+    double vecs = (k > 0 ? 3 : 1) + (k == i + 1 ? 0 : 1);
+    timed(4 + vt, vecs * nv * ws,
+          [&] { launch_mgs(s, c->n, vt, w, vprev, vk, k, i, c->st, c->partial, chk); });
+    c->counts[2]++;
+  }
 };
+
+// Resolve recorded launches into the per-class sums.  last_iter > 0: launches of
+// BiCGStab iterations after it returned at once (the solve had converged) and are
+// dropped; half: iteration last_iter ended with a half-step exit.
+void prof_resolve(asam_sound_ctx* c, bool half, int last_iter = 0) {
+  for (auto& r : c->prof_pending) {
+    if (last_iter > 0 && r.iter > last_iter) continue;
+    const bool in_last = last_iter > 0 && r.iter == last_iter;
+    if (half && in_last && r.tag == 1) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.e0, r.e1) != hipSuccess) continue;
+    c->prof_ms[r.cls] += ms;
+    c->prof_bytes[r.cls] += (half && in_last && r.tag == 2) ? r.half_bytes : r.bytes;
+    c->prof_launches[r.cls] += 1;
+  }
+  c->prof_pending.clear();
+  c->ev_next = 0;
+}
 
 int sync_state(asam_sound_ctx* c) {
   HIPCHK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(KState), hipMemcpyDeviceToHost, c->stream));
@@ -332,38 +478,45 @@ void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v
   // p = r + beta (p - omega v) (p = r at iter 1); pHat = PSolve(p)
   e.line(prec, LINE_BICG_P, r, p, v, ph, nullptr, &st->done);
   // v = MatVec(pHat); alpha = rho / (rTilde.v)
-  e.apply(prec, vt, vt, vt, ph, v, nullptr, nullptr, rt, DOT_YU, &st->done);
-  e.fin(FIN_BICG_ALPHA, 2, apply_blocks(g));
-  // s = r - alpha v (into r); |s|; rHat = PSolve(s)
-  e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done);
-  e.fin(FIN_BICG_HALF, 1, line_blocks(g));
-  // t = MatVec(rHat); omega = (t.s)/(t.t)
-  e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->pad);
-  e.fin(FIN_BICG_OMEGA, 2, apply_blocks(g));
-  // x += alpha pHat + omega rHat; r = s - omega t; |r|, rTilde.r
-  launch_bicg_update(e.s, c->n, vt, x, ph, rh, r, t, rt, st, c->partial);
-  c->counts[2]++;
-  e.fin(FIN_BICG_END, 2, blas_blocks(c->n));
+  e.apply(prec, vt, vt, vt, ph, v, nullptr, nullptr, rt, DOT_YU, &st->done, nullptr, nullptr, 0,
+          FIN_BICG_ALPHA, 2);
+  // s = r - alpha v (into r); |s| (partial slot 2); rHat = PSolve(s)
+  e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done, 2);
+  // t = MatVec(rHat); t.t, t.s; then the half-step exit test on |s| or omega = (t.s)/(t.t)
+  e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->done);
+  e.fin(FIN_BICG_HALF_OMEGA, 3, apply_blocks(g), 0, 0, 0, line_blocks(g));
+  // x += alpha pHat + omega rHat; r = s - omega t (half exit: x += alpha pHat);
+  // |r|, rTilde.r; exit tests, beta
+  e.bicg_update(vt, x, ph, rh, r, t, rt);
 }
 
 // Runs BiCGStab iterations until the device reports done; returns 0 or error.
 template <typename VT>
 int bicg_loop(Enq& e, int prec, int maxiter, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v, VT* rh,
-              VT* t, int* iters_out) {
+              VT* t, int* iters_out, int first_chunk) {
   asam_sound_ctx* c = e.c;
   int issued = 0;
-  int chunk = c->last_iters > 0 ? c->last_iters : 4;
+  // First chunk: the iteration count the same solve needed last time; then one at a time.
+  int chunk = first_chunk > 0 ? first_chunk : 4;
   if (chunk > maxiter) chunk = maxiter;
   while (issued < maxiter) {
     int q = chunk;
     if (issued + q > maxiter) q = maxiter - issued;
-    for (int k = 0; k < q; ++k) bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
+    for (int k = 0; k < q; ++k) {
+      e.iter = issued + k + 1;
+      bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
+    }
     issued += q;
     int rc = sync_state(c);
     if (rc) return rc;
+    if (c->prof_on) {
+      const bool dn = c->st_host->done != 0;
+      prof_resolve(c, dn && c->st_host->half != 0, dn ? c->st_host->iter : 0);
+    }
     if (c->st_host->done) break;
-    chunk = 2;
+    chunk = 1;
   }
+  e.iter = 0;  // launches after the loop belong to no BiCGStab iteration
   *iters_out = issued;
   return 0;
 }
@@ -380,12 +533,13 @@ static int solve_bicg_fp64(asam_sound_ctx* c, int tr, double* x, const double* b
   int rc = upload_state(c);
   if (rc) return rc;
   // r = b - MatVec(x); rTilde = r; rNorm0 = |r|
-  e.apply(0, 0, 0, 0, x, c->r, b, c->rt, nullptr, DOT_YY, nullptr);
-  e.fin(FIN_BICG_INIT, 1, apply_blocks(c->g));
+  e.apply(0, 0, 0, 0, x, c->r, b, c->rt, nullptr, DOT_YY, nullptr, nullptr, nullptr, 0,
+          FIN_BICG_INIT, 1);
   int issued = 0;
   if (*maxiter > 0) {
+    e.iter = 0;
     rc = bicg_loop<double>(e, 0, *maxiter, x, c->r, c->rt, c->p, c->ph, c->v, c->rh, c->t,
-                           &issued);
+                           &issued, c->last_iters);
     if (rc) return rc;
   } else {
     rc = sync_state(c);
@@ -447,8 +601,7 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, 
     } else {
       e.apply(0, 0, 0, 0, x, c->gtmp, b, nullptr, nullptr, 0, &st->done);
       e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[0], nullptr, &st->done);
-      launch_mgs(e.s, n, 0, c->gv[0], nullptr, nullptr, 0, -1, st, c->partial, 1);
-      c->counts[2]++;
+      e.mgs(0, c->gv[0], nullptr, nullptr, 0, -1, 1);
       e.fin(FIN_GM_INIT, 1, blas_blocks(n));
     }
     for (int i = 0; i < m; ++i) {
@@ -462,17 +615,14 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, 
         e.apply(0, 0, 0, 0, c->gv[i], c->gtmp, nullptr, nullptr, nullptr, 0, &st->cycle_done);
         e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[i + 1], &st->vscale[i],
                &st->cycle_done);
-        launch_mgs(e.s, n, 0, c->gv[i + 1], nullptr, c->gv[0], 0, i, st, c->partial);
-        c->counts[2]++;
+        e.mgs(0, c->gv[i + 1], nullptr, c->gv[0], 0, i);
         e.fin(FIN_GM_H, 1, blas_blocks(n), 0, i);
       }
       for (int k = 1; k <= i; ++k) {
-        launch_mgs(e.s, n, 0, c->gv[i + 1], c->gv[k - 1], c->gv[k], k, i, st, c->partial);
-        c->counts[2]++;
+        e.mgs(0, c->gv[i + 1], c->gv[k - 1], c->gv[k], k, i);
         e.fin(FIN_GM_H, 1, blas_blocks(n), k, i);
       }
-      launch_mgs(e.s, n, 0, c->gv[i + 1], c->gv[i], nullptr, i + 1, i, st, c->partial);
-      c->counts[2]++;
+      e.mgs(0, c->gv[i + 1], c->gv[i], nullptr, i + 1, i);
       e.fin(FIN_GM_LAST, 1, blas_blocks(n), 0, i);
     }
     launch_gm_backsolve(e.s, st);
@@ -513,24 +663,26 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   bool converged = false;
   for (outer = 0; outer <= c->ctl.max_outer; ++outer) {
     if (outer == 0) {
-      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr);
+      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, nullptr, nullptr,
+              0, FIN_MIXED_OUTER, 1);
     } else {
-      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, c->d32, xnext);
+      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, c->d32, xnext, 0,
+              FIN_MIXED_OUTER, 1);
       double* tmp = xcur; xcur = xnext; xnext = tmp;
     }
-    launch_mixed_outer(e.s, apply_blocks(c->g), c->partial, st);
-    c->counts[3]++;
     rc = sync_state(c);
     if (rc) return rc;
     if (hs->outer_done) { converged = true; break; }
     if (outer == c->ctl.max_outer) break;
-    launch_fill(e.s, c->n, 1, c->d32, 0.0);
+    e.timed(5, 4.0 * c->n, [&] { launch_fill(e.s, c->n, 1, c->d32, 0.0); });
     int issued = 0;
+    const int hist = outer < 32 ? c->inner_hist[outer] : 0;
     rc = bicg_loop<float>(e, 1, c->ctl.max_inner, c->d32, c->r32, c->rt32, c->p32, c->ph32,
-                          c->v32, c->rh32, c->t32, &issued);
+                          c->v32, c->rh32, c->t32, &issued, hist > 0 ? hist : 4);
     if (rc) return rc;
     int res = hs->result;
     int it = res < 0 ? -res : (hs->status == 1 ? res / 2 : res);
+    if (outer < 32) c->inner_hist[outer] = it > 0 ? it : 1;
     inner_total += it;
   }
   if (xcur != x) HIPCHK(c, hipMemcpyAsync(x, xcur, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
@@ -568,6 +720,7 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
   if (rc) return rc;
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
+  if (c->prof_on) prof_resolve(c, false);
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   tmp.ms = ms;
@@ -772,6 +925,44 @@ extern "C" int asam_sound_time_kernel(asam_sound_ctx* c, int which, int fp32, in
   float t = 0.f;
   HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
   HIPCHK(c, hipGetLastError());
+  *ms = (double)t / reps;
+  return 0;
+}
+
+extern "C" int asam_sound_profile_enable(asam_sound_ctx* c, int on) {
+  if (!c) return ASAM_E_INVALID;
+  c->prof_on = on ? 1 : 0;
+  return 0;
+}
+
+extern "C" int asam_sound_profile_read(asam_sound_ctx* c, double ms[8], double bytes[8],
+                                       int launches[8]) {
+  if (!c || !ms || !bytes || !launches) return ASAM_E_INVALID;
+  for (int q = 0; q < 8; ++q) {
+    ms[q] = c->prof_ms[q]; bytes[q] = c->prof_bytes[q]; launches[q] = c->prof_launches[q];
+    c->prof_ms[q] = 0; c->prof_bytes[q] = 0; c->prof_launches[q] = 0;
+  }
+  return 0;
+}
+
+extern "C" int asam_sound_time_copy(asam_sound_ctx* c, size_t bytes, int reps, double* ms) {
+  if (!c || !ms || reps < 1 || bytes < 16) return ASAM_E_INVALID;
+  HIPCHK(c, hipSetDevice(c->device));
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(c, hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) { (void)hipFree(a); return fail(c, ASAM_E_NOMEM, "copy"); }
+  const long n16 = (long)(bytes / 16);
+  (void)hipMemsetAsync(a, 0, bytes, c->stream);
+  launch_copy16(c->stream, n16, a, b);
+  (void)hipEventRecord(c->ev0, c->stream);
+  for (int q = 0; q < reps; ++q) launch_copy16(c->stream, n16, a, b);
+  (void)hipEventRecord(c->ev1, c->stream);
+  hipError_t e = hipEventSynchronize(c->ev1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev0, c->ev1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(c, ASAM_E_HIP, hipGetErrorString(e));
   *ms = (double)t / reps;
   return 0;
 }

@@ -45,6 +45,15 @@ class Info(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("apply_variant", C.c_int), ("apply_wg", C.c_int), ("line_window", C.c_int),
+                ("line_lds", C.c_int), ("fused_tail", C.c_int), ("apply_xcd", C.c_int),
+                ("apply_bx", C.c_int), ("line_tiled", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class AsamError(RuntimeError):
     pass
 
@@ -98,6 +107,11 @@ def load(path: str = LIB_PATH):
         "asam_dev_mass": (i, [_vp, _vp, _vp]),
         "asam_sound_time_kernel": (i, [_vp, i, i, i, dp]),
         "asam_sound_last_counts": (i, [_vp, ip]),
+        "asam_sound_profile_enable": (i, [_vp, i]),
+        "asam_sound_get_tuning": (i, [_vp, C.POINTER(Tuning)]),
+        "asam_sound_time_copy": (i, [_vp, C.c_size_t, i, dp]),
+        "asam_sound_set_tuning": (i, [_vp, C.POINTER(Tuning)]),
+        "asam_sound_profile_read": (i, [_vp, dp, dp, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -211,6 +225,18 @@ class SoundSolver:
                            max_inner, max_outer)
         self._chk(self.L.asam_sound_set_control(self.h, C.byref(self.ctl)))
 
+    def get_tuning(self) -> dict:
+        t = Tuning()
+        self._chk(self.L.asam_sound_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        self._chk(self.L.asam_sound_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._chk(self.L.asam_sound_set_tuning(self.h, C.byref(t)))
+
     # ---------------------------------------------------------------- solve
     def SolveSound(self, x, b, MaxIter, Tol):
         """Reference semantics: solves MatVec1(x) = -A x = b in place of x.
@@ -298,6 +324,24 @@ class SoundSolver:
     def time_kernel(self, which: int, fp32: bool, reps: int = 20) -> float:
         ms = C.c_double(0.0)
         self._chk(self.L.asam_sound_time_kernel(self.h, which, int(fp32), reps, C.byref(ms)))
+        return ms.value
+
+    PROFILE_CLASSES = ("apply_fp64", "apply_fp32", "line_fp64", "line_fp32", "blas_fp64",
+                       "blas_fp32", "reduce", "other")
+
+    def profile(self, on: bool = True):
+        self._chk(self.L.asam_sound_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        """Per kernel class: (ms summed, algorithmic bytes summed, launches) since last read."""
+        ms, by, ln = (C.c_double * 8)(), (C.c_double * 8)(), (C.c_int * 8)()
+        self._chk(self.L.asam_sound_profile_read(self.h, ms, by, ln))
+        return {k: {"ms": ms[q], "bytes": by[q], "launches": ln[q]}
+                for q, k in enumerate(self.PROFILE_CLASSES)}
+
+    def time_copy(self, nbytes: int, reps: int = 20) -> float:
+        ms = C.c_double(0.0)
+        self._chk(self.L.asam_sound_time_copy(self.h, nbytes, reps, C.byref(ms)))
         return ms.value
 
     def last_counts(self):

@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the sound stage solve on one MI355X.
+
+A step is one SolveSound call (x = 0 initial guess, as ProjectVelFace does,
+reference SRC/ADVECT/IntSub_Mod.f90:46-56) on the 256x256x64 cut-cell system
+(Witch-of-Agnesi ridge, h=1 km, a=5 km; dx=dy=500 m, dz=200 m; theta=300 K
+hydrostatic + 2 K bubble; b ~ N(0,1) seeded), buoyancy-coupled MultiTriTB
+operator (2 unknowns per cell), BiCGStab with the vertical-line preconditioner,
+to 1e-8 relative residual.  The headline solve is the FP32-inner / FP64-refined
+one; the plain FP64 GPU solve is timed beside it on the same resident inputs.
+
+Prints one JSON line.  Live kernel timing: every launch of the timed solves is
+bracketed by HIP events on the solver's stream (libasam_sound profiling), and
+each launch's algorithmic bytes are accounted, so roofline.achieved is measured,
+not modelled.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--np1", type=int, default=2, help="1: MultiTriT, 2: MultiTriTB")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-hbm", action="store_true", help="skip the 512x512x96 kernel roofline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--kernel-probe", type=int, default=0, metavar="CFG",
+                    help="time the copy, apply and line kernels in isolation on config CFG "
+                         "(2: 256x256x64, 3: 512x512x96) and exit (profiler runs)")
+    ap.add_argument("--reps", type=int, default=10)
+    return ap.parse_args()
+
+
+def kernel_probe(cfg, np1, reps):
+    """Isolated launches of the dominant kernels plus a streaming copy of known size."""
+    from asam_amd import sound, workload
+    p = workload.config(cfg, np1=np1)
+    s = sound.SoundSolver.from_problem(p)
+    N = p.grid.ncell
+    out = {"config": cfg, "np1": np1, "cells": N}
+    copy_bytes = 1 << 30
+    ms = s.time_copy(copy_bytes, reps)
+    out["copy_1GiB"] = {"avg_launch_us": round(ms * 1e3, 1),
+                        "GBs": round(2 * copy_bytes / (ms * 1e-3) / 1e9, 1)}
+    for which, name in ((0, "apply"), (1, "line")):
+        for fp32 in (True, False):
+            ms = s.time_kernel(which, fp32, reps)
+            w = 4 if fp32 else 8
+            words = (7 if np1 == 1 else 13) if which == 0 else (5 if np1 == 1 else 16)
+            out[f"{name}_{'fp32' if fp32 else 'fp64'}"] = {
+                "avg_launch_us": round(ms * 1e3, 1), "algorithmic_bytes": words * w * N,
+                "GBs": round(words * w * N / (ms * 1e-3) / 1e9, 1)}
+    s.close()
+    print(json.dumps(out))
+
+
+def timed_solves(s, xbuf, bbuf, steps, warmup, tol, profile):
+    for _ in range(warmup):
+        s._chk(s.L.asam_dev_memset(s.h, xbuf.ptr, 0, s.n * 8))
+        s.solve_device(xbuf, bbuf, 500, tol)
+    s.profile(profile)
+    s.profile_read()
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s._chk(s.L.asam_dev_memset(s.h, xbuf.ptr, 0, s.n * 8))
+        _, _, info = s.solve_device(xbuf, bbuf, 500, tol)
+        infos.append(info.as_dict())
+    t1 = time.perf_counter()  # every solve returns after a stream synchronisation
+    prof = s.profile_read()
+    s.profile(False)
+    return (t1 - t0) / steps, infos, prof
+
+
+def roofline_from_profile(prof, np1):
+    # dominant kernel class = most device time inside the timed region
+    best = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    name, d = best
+    if d["launches"] == 0 or d["ms"] <= 0:
+        return None, name
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": name, "avg_launch_us": round(avg_ms * 1e3, 2),
+            "bytes_per_launch": int(bytes_per_launch)}, name
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (scaled by
+    the copy-kernel factor), recorded in asam_amd/traffic.json; None if not measured."""
+    path = os.path.join(ROOT, "asam_amd", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(kernel)
+    except Exception:
+        return None
+
+
+def cpu_baseline(p, tol, seconds):
+    from oracle import pyoracle
+    threads = min(os.cpu_count() or 1, 16)
+    pyoracle.set_threads(threads)
+    orc = pyoracle.Oracle(p)
+    t0 = time.perf_counter()
+    x, mi, tl, st = orc.bicgstab(p.b, 500, tol)
+    t1 = time.perf_counter()
+    times = [t1 - t0]
+    while sum(times) < seconds and len(times) < 8:
+        t0 = time.perf_counter()
+        orc.bicgstab(p.b, 500, tol)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": round(1.0 / med, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"{len(times)} full FP64 BiCGStab solves of the same 256x256x64 system "
+                      f"(C restatement of the reference algorithm, OpenMP over columns), "
+                      f"median {med:.2f} s"}, x, mi, orc
+
+
+def hbm_roofline(np1):
+    """Apply/line kernel bandwidth at 512x512x96 (working set well beyond the 256 MB
+    Infinity Cache), FP32 and FP64, isolated launches timed with HIP events."""
+    from asam_amd import sound, workload
+    p = workload.config(3, np1=np1)
+    s = sound.SoundSolver.from_problem(p)
+    N = p.grid.ncell
+    out = {}
+    for which, name in ((0, "apply"), (1, "line")):
+        for fp32 in (True, False):
+            ms = s.time_kernel(which, fp32, reps=20)
+            w = 4 if fp32 else 8
+            words = (7 if np1 == 1 else 13) if which == 0 else (5 if np1 == 1 else 16)
+            gbs = words * w * N / (ms * 1e-3) / 1e9
+            out[f"{name}_{'fp32' if fp32 else 'fp64'}"] = {
+                "avg_launch_us": round(ms * 1e3, 1), "achieved_GBs": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    s.close()
+    return out
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the sound stage solve is not sharded across GPUs "
+              "in this build (see DESIGN.md for the horizontal-tile sharding plan).",
+              file=sys.stderr)
+        sys.exit(2)
+    if a.kernel_probe:
+        kernel_probe(a.kernel_probe, a.np1, a.reps)
+        return
+    from asam_amd import sound, workload
+    p = workload.config(2, np1=a.np1)
+    s = sound.SoundSolver.from_problem(p)
+    xbuf = s.vec()
+    bbuf = s.vec(p.b)
+
+    # Timed region: K solves, uninstrumented (value, ms_per_step); then the same K solves
+    # with every kernel launch bracketed by HIP events (per-kernel durations and bytes for
+    # the roofline).  Both runs issue identical launch sequences.
+    s.set_control(precision="FP64")
+    t64, infos64, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
+    t64_i, _, prof64 = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
+    x64 = xbuf.download()
+
+    s.set_control(precision="MIXED")
+    tmx, infosmx, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
+    tmx_i, _, profmx = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
+    xmx = xbuf.download()
+
+    # residual of the mixed solution with the FP64 GPU operator (size-independent check)
+    rbuf = s.vec()
+    s.apply_device(rbuf, xbuf)
+    r = p.b - rbuf.download()
+    true_rel = float(np.linalg.norm(r) / np.linalg.norm(p.b))
+    diff_vs_fp64_gpu = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+
+    roof, dom = roofline_from_profile(profmx, a.np1)
+    if roof is not None:
+        roof["traffic"] = read_traffic(dom)
+    line = {
+        "metric": "Krylov solves/s (FP32-inner + FP64-refined BiCGStab to 1e-8), 256x256x64 cut-cell",
+        "value": round(1.0 / tmx, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(tmx * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 residual, dots, refinement",
+        "data": "synthetic (seeded Agnesi ridge, hydrostatic bubble state, b~N(0,1))",
+        "config": {"workload": f"256x256x64 cut-cell {'MultiTriTB (buoyancy, 2 unknowns/cell)' if a.np1 == 2 else 'MultiTriT'} "
+                               f"stage solve, BiCGStab + vertical-line precond, tol {a.tol:g}",
+                   "cells": p.grid.ncell, "unknowns": p.n, "parallelism": "1 GPU"},
+        "roofline": roof,
+        "fp64": {"solves_per_s": round(1.0 / t64, 3), "ms_per_step": round(t64 * 1e3, 4),
+                 "iterations": infos64[-1]["iterations"],
+                 "instrumented_ms_per_step": round(t64_i * 1e3, 4)},
+        "mixed": {"inner_iterations": infosmx[-1]["iterations"],
+                  "refinement_steps": infosmx[-1]["restarts"],
+                  "true_rel_residual": true_rel,
+                  "rel_diff_vs_fp64_gpu": diff_vs_fp64_gpu,
+                  "instrumented_ms_per_step": round(tmx_i * 1e3, 4)},
+        "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+        "kernel_timing": "HIP events around every launch of the instrumented replay of the timed solves",
+        "kernels_mixed": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
+                              "launches_per_solve": v["launches"] / a.steps,
+                              "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                          for k, v in profmx.items() if v["launches"]},
+        "kernels_fp64": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
+                             "launches_per_solve": v["launches"] / a.steps,
+                             "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
+                         for k, v in prof64.items() if v["launches"]},
+    }
+    s.close()
+    if not a.no_hbm:
+        line["roofline_512x512x96"] = hbm_roofline(a.np1)
+    if not a.no_cpu:
+        cb, xo, mo, orc = cpu_baseline(p, a.tol, a.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["mixed"]["err_vs_fp64_oracle"] = float(np.linalg.norm(xmx - xo) / np.linalg.norm(xo))
+        line["fp64"]["err_vs_fp64_oracle"] = float(np.linalg.norm(x64 - xo) / np.linalg.norm(xo))
+        line["fp64"]["oracle_maxiter_out"] = mo
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

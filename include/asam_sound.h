@@ -86,11 +86,26 @@ typedef struct {
   double ms;              /* device time of the solve (HIP events), milliseconds */
 } asam_sound_info;
 
+/* Launch-shape knobs of the kernels (no effect on results beyond round-off order of
+ * the reductions; the defaults are tuned for MI355X). */
+typedef struct {
+  int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread */
+  int apply_wg;      /* target workgroups per stencil-apply launch (256..16384) */
+  int line_window;   /* levels per load window of the line solve: 1, 4 or 8 */
+  int line_lds;      /* 1: stage the forward sweep in LDS, 0: in the output vector */
+  int fused_tail;    /* reserved, must be 0 (reductions end in a separate finalize launch) */
+  int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
+  int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
+  int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
+} asam_sound_tuning;
+
 /* ---- context ---------------------------------------------------------- */
 int asam_sound_create(asam_sound_ctx **ctx, const asam_sound_grid *grid);
 int asam_sound_destroy(asam_sound_ctx *ctx);
 int asam_sound_default_control(asam_sound_control *ctl);
 int asam_sound_set_control(asam_sound_ctx *ctx, const asam_sound_control *ctl);
+int asam_sound_get_tuning(const asam_sound_ctx *ctx, asam_sound_tuning *t);
+int asam_sound_set_tuning(asam_sound_ctx *ctx, const asam_sound_tuning *t);
 /* Last error message of this context (static storage of the context). */
 const char *asam_sound_last_error(const asam_sound_ctx *ctx);
 
@@ -170,8 +185,18 @@ int asam_dev_mass(asam_sound_ctx *ctx, double *y, const double *x);
  * stencil apply (which=0), line preconditioner (which=1) or fused BiCGStab update
  * (which=2), in FP64 (fp32=0) or FP32 (fp32=1), over reps launches. */
 int asam_sound_time_kernel(asam_sound_ctx *ctx, int which, int fp32, int reps, double *ms);
+/* Average time (ms) of a 16-byte-vector streaming copy of `bytes` bytes (reads bytes, writes
+ * bytes): the achievable-bandwidth and profiler-counter calibration kernel. */
+int asam_sound_time_copy(asam_sound_ctx *ctx, size_t bytes, int reps, double *ms);
 /* Kernel launch counts of the last solve, per kernel class (apply, line, blas, reduce). */
 int asam_sound_last_counts(const asam_sound_ctx *ctx, int counts[8]);
+/* Live per-kernel timing: when enabled, every launch of a solve is bracketed by HIP
+ * events on the context's stream and its algorithmic bytes (each distinct array element
+ * read or written once) are accounted.  Classes: 0 apply FP64, 1 apply FP32, 2 line FP64,
+ * 3 line FP32, 4 BLAS-1 FP64, 5 BLAS-1 FP32, 6 reduction tails, 7 other.
+ * asam_sound_profile_read returns the sums since the last read and resets them. */
+int asam_sound_profile_enable(asam_sound_ctx *ctx, int on);
+int asam_sound_profile_read(asam_sound_ctx *ctx, double ms[8], double bytes[8], int launches[8]);
 
 #ifdef __cplusplus
 }
