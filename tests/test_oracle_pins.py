@@ -1,0 +1,238 @@
+"""Pins of the CPU oracle that do not rely on the oracle's own formulas.
+
+The reference ships no tests or golden vectors for this path, so each part of the
+oracle is checked against something the mathematics fixes:
+  - the line (Thomas) solves against a library sparse solve of the same blocks;
+  - the apply against an explicit sparse matrix of the exported coefficients;
+  - the coefficients against conservation (face terms cancel pairwise), the Neumann
+    null vector 1/DUT, symmetry of A diag(1/DUT), closed-form cosine eigenpairs of the
+    uniform operator (Full and Dual forms), and a closed form of the gravity coupling;
+  - the Krylov solvers against a library direct solve, plus the reference's
+    MaxIter/Tol return conventions.
+A plausible mistake (a dropped factor 2, a swapped u/l, a sign of the gravity flux,
+an index off by one) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from asam_amd import workload as W
+
+BD = 0.4358665 * 2.0
+
+
+def tiny(np1, grad_full=1, nx=6, ny=5, nz=7, H=600.0, seed=3):
+    return W.make_problem(nx, ny, nz, 1000.0, 1000.0, 250.0, H=H, a=1500.0,
+                          ridge_xc=0.5 * nx * 1000.0, np1=np1, grad_full=grad_full, seed=seed)
+
+
+def live(p, v):
+    m = np.repeat((p.VolC.reshape(-1) > 0.0), p.np1)
+    return np.where(m, v, 0.0)
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+@pytest.mark.parametrize("grad_full", [1, 0])
+def test_apply_matches_explicit_matrix(np1, grad_full, oracle_mod):
+    p = tiny(np1, grad_full)
+    o = oracle_mod.Oracle(p)
+    A = oracle_mod.to_csr(o)
+    x = W.normal(o.n, 7)
+    assert np.allclose(o.apply(x), A @ x, rtol=0, atol=1e-13 * np.abs(A @ x).max())
+    assert np.allclose(o.matvec1_T(x), -(A.T @ x), rtol=0, atol=1e-13 * np.abs(A.T @ x).max())
+
+
+def column_blocks(p, A):
+    """Block-diagonal part of A keeping only couplings inside one vertical column."""
+    g = p.grid
+    N = g.ncell
+    col = np.arange(N) % (g.nx * g.ny)
+    col = np.repeat(col, p.np1)
+    A = A.tocoo()
+    keep = col[A.row] == col[A.col]
+    return sp.csr_matrix((A.data[keep], (A.row[keep], A.col[keep])), shape=A.shape)
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_line_solve_matches_library_solve(np1, oracle_mod):
+    p = tiny(np1)
+    o = oracle_mod.Oracle(p)
+    M = column_blocks(p, oracle_mod.to_csr(o))
+    x = live(p, W.normal(o.n, 9))
+    act = np.repeat(p.VolC.reshape(-1) > 0, np1)
+    ref = np.zeros_like(x)
+    Ma = M[act][:, act].tocsc()
+    ref[act] = spla.spsolve(Ma, x[act])
+    got = o.line_solve(x)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # PSolve1 semantics: -M^-1 x; transposed: -M^-T x
+    assert np.allclose(o.psolve1(x), -got, rtol=0, atol=1e-15 * np.abs(got).max())
+    refT = np.zeros_like(x)
+    refT[act] = spla.spsolve(Ma.T.tocsc(), x[act])
+    assert np.abs(-o.psolve1_T(x) - refT).max() <= 1e-12 * np.abs(refT).max()
+
+
+def test_thomas_matches_solve_banded(oracle_mod):
+    """One column (nx=ny=1): scalar Thomas against scipy.linalg.solve_banded."""
+    from scipy.linalg import solve_banded
+    p = tiny(1, nx=1, ny=1, nz=23, H=0.0)
+    o = oracle_mod.Oracle(p)
+    c = o.export()
+    n = p.grid.nz
+    ab = np.zeros((3, n))
+    ab[0, 1:] = c["uz"][:n - 1]
+    ab[1, :] = c["d"]
+    ab[2, :-1] = c["lz"][:n - 1]
+    x = W.normal(n, 5)
+    assert np.allclose(o.line_solve(x), solve_banded((1, 1), ab, x), rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("grad_full", [1, 0])
+def test_conservation_and_null_vector_T(grad_full, oracle_mod):
+    p = tiny(1, grad_full)
+    o = oracle_mod.Oracle(p)
+    _, _, _, DUT = o.weights()
+    V = p.VolC.reshape(-1)
+    x = W.normal(o.n, 4)
+    # face terms cancel pairwise: sum_c (A x)_c = sum_c V_c x_c (+ the 1e-40 dead-cell seed)
+    lhs = o.apply(x).sum()
+    rhs = (V * x).sum() + 1e-40 * x.sum()
+    assert abs(lhs - rhs) <= 1e-12 * np.abs(V * x).sum()
+    # Neumann null vector of the Laplacian part: A (1/DUT) = V/DUT
+    u = 1.0 / DUT.reshape(-1)
+    assert np.allclose(o.apply(u), V * u + 1e-40 * u, rtol=1e-12, atol=1e-12 * (V * u).max())
+
+
+def test_symmetry_of_scaled_operator(oracle_mod):
+    p = tiny(1)
+    o = oracle_mod.Oracle(p)
+    _, _, _, DUT = o.weights()
+    A = oracle_mod.to_csr(o) @ sp.diags(1.0 / DUT.reshape(-1))
+    assert abs(A - A.T).max() <= 1e-13 * abs(A).max()
+
+
+def test_conservation_TB(oracle_mod):
+    """Both continuity equations conserve: row sums of each component equal V x."""
+    p = tiny(2)
+    o = oracle_mod.Oracle(p)
+    V = p.VolC.reshape(-1)
+    x = W.normal(o.n, 6).reshape(-1, 2)
+    y = o.apply(x.reshape(-1)).reshape(-1, 2)
+    for comp in (0, 1):
+        rhs = (V * x[:, comp]).sum() + 1e-40 * x[:, comp].sum()
+        assert abs(y[:, comp].sum() - rhs) <= 1e-12 * np.abs(V * x[:, comp]).sum()
+
+
+def uniform_problem(np1, grad_full, nx=8, ny=6, nz=5, dx=1000.0, dy=800.0, dz=250.0):
+    p = W.make_problem(nx, ny, nz, dx, dy, dz, H=0.0, np1=np1, grad_full=grad_full, seed=1)
+    th, dut = 300.0, 400.0
+    g = p.grid
+    DTU = np.full((g.nz, g.ny, g.nx + 1), th)
+    DTV = np.full((g.nz, g.ny + 1, g.nx), th)
+    DTW = np.full((g.nz + 1, g.ny, g.nx), th)
+    DUT = np.full((g.nz, g.ny, g.nx), dut)
+    return p, (DTU, DTV, DTW, DUT), th, dut
+
+
+@pytest.mark.parametrize("grad_full", [1, 0])
+def test_closed_form_cosine_eigenpairs(grad_full, oracle_mod):
+    """Uniform state, no terrain: A v = lambda v for Neumann cosine modes with
+    lambda = V + DUT sum_d kappa_d 2 (1 - cos(k_d pi / n_d)),
+    kappa_d = (b0 dt)^2 F_d theta / d_d (Full) or (b0 dt)^2 F_d^2 theta / V (Dual)."""
+    p, w, th, dut = uniform_problem(1, grad_full)
+    o = oracle_mod.Oracle(p, weights=w)
+    g = p.grid
+    V = g.dx * g.dy * g.dz
+    F = {"x": g.dy * g.dz, "y": g.dx * g.dz, "z": g.dx * g.dy}
+    d = {"x": g.dx, "y": g.dy, "z": g.dz}
+    kap = {a: (BD ** 2 * F[a] * th / d[a]) if grad_full else (BD ** 2 * F[a] ** 2 * th / V)
+           for a in "xyz"}
+    ii = np.arange(g.nx)[None, None, :]
+    jj = np.arange(g.ny)[None, :, None]
+    kk = np.arange(g.nz)[:, None, None]
+    for (kx, ky, kz) in ((0, 0, 0), (1, 0, 0), (0, 2, 0), (0, 0, 3), (3, 1, 2), (7, 5, 4)):
+        v = (np.cos(kx * math.pi * (ii + 0.5) / g.nx) * np.cos(ky * math.pi * (jj + 0.5) / g.ny)
+             * np.cos(kz * math.pi * (kk + 0.5) / g.nz)).reshape(-1)
+        lam = V + dut * sum(kap[a] * 2 * (1 - math.cos(kq * math.pi / n))
+                            for a, kq, n in (("x", kx, g.nx), ("y", ky, g.ny), ("z", kz, g.nz)))
+        Av = o.apply(v)
+        assert np.allclose(Av, lam * v + 1e-40 * v, rtol=0, atol=1e-11 * lam), (kx, ky, kz)
+
+
+def test_gravity_closed_form_TB(oracle_mod):
+    """x = (0, c): interior rows see V c, the bottom/top cells V c -/+ g (b0 dt)^2 F c;
+    row 1 carries the same gravity flux times theta."""
+    p, w, th, dut = uniform_problem(2, 1, nz=6)
+    o = oracle_mod.Oracle(p, weights=w)
+    g = p.grid
+    V = g.dx * g.dy * g.dz
+    Fz = g.dx * g.dy
+    G = W.GRAV * BD ** 2 * Fz
+    x = np.zeros((g.ncell, 2))
+    x[:, 1] = 1.0
+    y = o.apply(x.reshape(-1)).reshape(g.nz, g.ny, g.nx, 2)
+    expect2 = np.full(g.nz, V)
+    expect2[0] -= G
+    expect2[-1] += G
+    expect1 = np.zeros(g.nz)
+    expect1[0] -= th * G
+    expect1[-1] += th * G
+    for k in range(g.nz):
+        assert np.allclose(y[k, :, :, 1], expect2[k] + 1e-40, rtol=1e-13)
+        assert np.allclose(y[k, :, :, 0], expect1[k], rtol=1e-13, atol=1e-9 * V)
+
+
+def test_TB_without_gravity_reduces_to_T(oracle_mod):
+    """grav = 0: the rho*theta row block of MultiTriTB is the MultiTriT operator (Full
+    form: FG/(Va+Vb) equals 1/(da+db))."""
+    pT = tiny(1)
+    pB = tiny(2)
+    pB.grav = 0.0
+    oT, oB = oracle_mod.Oracle(pT), oracle_mod.Oracle(pB)
+    x1 = live(pT, W.normal(pT.n, 8))
+    xb = np.zeros((pT.n, 2))
+    xb[:, 0] = x1
+    yB = oB.apply(xb.reshape(-1)).reshape(-1, 2)[:, 0]
+    yT = oT.apply(x1)
+    assert np.allclose(yB, yT, rtol=1e-12, atol=1e-12 * np.abs(yT).max())
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_krylov_solvers_match_direct_solve(np1, oracle_mod):
+    p = tiny(np1)
+    o = oracle_mod.Oracle(p)
+    A = oracle_mod.to_csr(o)
+    act = np.repeat(p.VolC.reshape(-1) > 0, np1)
+    Aa = (-A)[act][:, act].tocsc()
+    ref = np.zeros(o.n)
+    ref[act] = spla.spsolve(Aa, p.b[act])
+    refT = np.zeros(o.n)
+    refT[act] = spla.spsolve(Aa.T.tocsc(), p.b[act])
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)  # noqa: E731
+    x, mi, tl, st = o.bicgstab(p.b, 200, 1e-13)
+    assert st == 0 and rel(x, ref) < 1e-10
+    x, it, rr = o.fgmres(p.b, 30, 1e-13, 10)
+    assert rel(x, ref) < 1e-10
+    x, it, rr = o.gmres(p.b, 30, 1e-13, 10)
+    assert rel(x, ref) < 1e-10
+    x, mi, tl, st = o.bicgstab(p.b, 200, 1e-13, transpose=True)
+    assert rel(x, refT) < 1e-10
+
+
+def test_bicgstab_reference_return_conventions(oracle_mod):
+    p = tiny(1)
+    o = oracle_mod.Oracle(p)
+    # b = 0: rho == 0 at iteration 1 -> MaxIter = 1, Tol = 0 (SpMatrix4CellPar_Mod.f90:286-290)
+    x, mi, tl, st = o.bicgstab(np.zeros(o.n), 50, 1e-8)
+    assert mi == 1 and tl == 0.0 and np.all(x == 0)
+    # not converged: MaxIter = 2*Iter, Tol = final |r| (:333-337)
+    x, mi, tl, st = o.bicgstab(p.b, 2, 1e-15)
+    assert mi == 4 and tl > 0 and st == 1
+    # converged: |MaxIter| = iterations and Tol = |r| <= tol*|r0|
+    x, mi, tl, st = o.bicgstab(p.b, 200, 1e-8)
+    r0 = np.linalg.norm(p.b)
+    assert st == 0 and tl <= 1e-8 * r0 and abs(mi) >= 1
+    assert np.isclose(np.linalg.norm(p.b - o.matvec1(x)), tl, rtol=1e-6)
