@@ -133,9 +133,14 @@ __global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
             h += kz[c] * (x0 - LX(mU));
             s += d0 * h;
           }
-          // Op = -A x; residual form y = b - Op(x) = b + A x
-          if constexpr ((FL & AF_B) != 0) yv1 = a.b[c] + (double)s;
-          else yv1 = -(double)s;
+          // Op = -A x; residual form y = b - Op(x) = b + A x, zero on dead cells
+          if constexpr ((FL & AF_B) != 0) {
+            const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
+                                                 kz[mD] + kz[c] == (CT)0;
+            yv1 = dead ? 0.0 : a.b[c] + (double)s;
+          } else {
+            yv1 = -(double)s;
+          }
           if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
           a.y[c] = (VO)yv1;
           if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv1;
@@ -203,7 +208,10 @@ __global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
           }
           if constexpr ((FL & AF_B) != 0) {
             const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
-            yv1 = bb.x + (double)s1; yv2 = bb.y + (double)s2;
+            const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
+                                                 kz[mD] + kz[c] == (CT)0;
+            yv1 = dead ? 0.0 : bb.x + (double)s1;
+            yv2 = dead ? 0.0 : bb.y + (double)s2;
           } else {
             yv1 = -(double)s1; yv2 = -(double)s2;
           }
@@ -289,8 +297,13 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
             s += d0 * h;
           }
           double yv;
-          if constexpr ((FL & AF_B) != 0) yv = a.b[c] + (double)s;
-          else yv = -(double)s;
+          if constexpr ((FL & AF_B) != 0) {
+            const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 ==
+                                                     (CT)0;
+            yv = dead ? 0.0 : a.b[c] + (double)s;
+          } else {
+            yv = -(double)s;
+          }
           if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
           a.y[c] = (VO)yv;
           if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv;
@@ -369,7 +382,9 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
           double yv1, yv2;
           if constexpr ((FL & AF_B) != 0) {
             const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
-            yv1 = bb.x + (double)s1; yv2 = bb.y + (double)s2;
+            const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 == (CT)0;
+            yv1 = dead ? 0.0 : bb.x + (double)s1;
+            yv2 = dead ? 0.0 : bb.y + (double)s2;
           } else {
             yv1 = -(double)s1; yv2 = -(double)s2;
           }
