@@ -1,0 +1,129 @@
+!> Drop-in replacement of ASAM's sound-solve entry points for a Fortran host.
+!>
+!> Keeps the reference's call interface for the stage solve:
+!>   SolveSound(x,b,MaxIter,Tol)   (reference SRC/SOLU/Iter_Mod.f90:9-30)
+!>   MatVec1(b,x), PSolve1(b,x)    (reference SRC/MATRIX/MatVec_Mod.f90:8, PreCond_Mod.f90:1555)
+!> and the once-per-step operator set-up JacAccGrav (SRC/MULTI/JacAccGrav_Mod.f90:304) as
+!> AsamJacAccGrav, with x and b carrying the reference's PressureVelocity%p component,
+!> p(1:np1,nx,ny,nz).  The arithmetic runs on the GPU through libasam_sound.
+module AsamSound_Mod
+  use iso_c_binding
+  use asam_sound_iface
+  implicit none
+  private
+  public :: PressureVelocity, AsamSoundInit, AsamJacAccGrav, SolveSound, SolveSoundT, &
+            MatVec1, PSolve1, AsamSoundFinalize, AsamLastInfo, AsamStatus
+
+  !> The p component of the reference's TYPE(PressureVelocity) (VecPrVel_Mod.f90:11-18);
+  !> one closed block, so the interface velocities are absent.
+  type PressureVelocity
+    real(c_double), pointer, contiguous :: p(:,:,:,:) => null()
+  end type PressureVelocity
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  type(asam_sound_info), save :: AsamLastInfo
+  integer, save :: AsamStatus = 0
+
+contains
+
+  !> Create the solver context for one block (nx,ny,nz interior cells).
+  subroutine AsamSoundInit(nx, ny, nz, np1, GradFull, beta0dtP, GravComp, FacAnela, &
+                           MethSound, StepGMRES, Mixed)
+    integer, intent(in) :: nx, ny, nz, np1
+    logical, intent(in) :: GradFull
+    real(c_double), intent(in) :: beta0dtP, GravComp, FacAnela
+    character(len=*), intent(in) :: MethSound
+    integer, intent(in) :: StepGMRES
+    logical, intent(in) :: Mixed
+    type(asam_sound_grid) :: g
+    type(asam_sound_control) :: ctl
+    integer(c_int) :: rc
+    if (c_associated(ctx)) call AsamSoundFinalize()
+    g%nx = nx; g%ny = ny; g%nz = nz; g%np1 = np1
+    g%grad_full = merge(1, 0, GradFull)
+    g%beta0_dt = beta0dtP; g%grav = GravComp; g%fac_anela = FacAnela
+    rc = asam_sound_create(ctx, g)
+    call check(rc, 'asam_sound_create')
+    rc = asam_sound_default_control(ctl)
+    select case (trim(MethSound))
+    case ('FGMRES')
+      ctl%method = ASAM_METHOD_FGMRES
+    case ('GMRES')
+      ctl%method = ASAM_METHOD_GMRES
+    case default
+      ctl%method = ASAM_METHOD_BICGSTAB
+    end select
+    ctl%step_gmres = StepGMRES
+    if (Mixed) ctl%precision = ASAM_PREC_MIXED
+    rc = asam_sound_set_control(ctx, ctl)
+    call check(rc, 'asam_sound_set_control')
+  end subroutine AsamSoundInit
+
+  !> Operator set-up from geometry and thermodynamic state (JacAccGrav).
+  subroutine AsamJacAccGrav(FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T)
+    real(c_double), intent(in) :: FU(:,:,:), FV(:,:,:), FW(:,:,:), VolC(:,:,:)
+    real(c_double), intent(in) :: dx(:), dy(:), dz(:)
+    real(c_double), intent(in) :: Rho(:,:,:), RhoTheta(:,:,:), T(:,:,:)
+    integer(c_int) :: rc
+    rc = asam_sound_setup(ctx, FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T)
+    call check(rc, 'asam_sound_setup')
+  end subroutine AsamJacAccGrav
+
+  !> SolveSound(x,b,MaxIter,Tol): solves MatVec1(x) = -A x = b (Iter_Mod.f90:9-30).
+  subroutine SolveSound(x, b, MaxIter, Tol)
+    type(PressureVelocity), dimension(:) :: b, x
+    integer :: MaxIter
+    real(c_double) :: Tol
+    integer(c_int) :: rc, mi
+    mi = MaxIter
+    rc = asam_sound_solve(ctx, x(1)%p, b(1)%p, mi, Tol, AsamLastInfo)
+    call check(rc, 'asam_sound_solve')
+    MaxIter = mi
+  end subroutine SolveSound
+
+  !> Adjoint of SolveSound: solves (-A)^T x = b with the transposed preconditioner.
+  subroutine SolveSoundT(x, b, MaxIter, Tol)
+    type(PressureVelocity), dimension(:) :: b, x
+    integer :: MaxIter
+    real(c_double) :: Tol
+    integer(c_int) :: rc, mi
+    mi = MaxIter
+    rc = asam_sound_solve_T(ctx, x(1)%p, b(1)%p, mi, Tol, AsamLastInfo)
+    call check(rc, 'asam_sound_solve_T')
+    MaxIter = mi
+  end subroutine SolveSoundT
+
+  !> b = MatVec1(x) = -A x (MatVec_Mod.f90:8-18).
+  subroutine MatVec1(b, x)
+    type(PressureVelocity), dimension(:) :: b, x
+    integer(c_int) :: rc
+    rc = asam_sound_apply(ctx, b(1)%p, x(1)%p)
+    call check(rc, 'asam_sound_apply')
+  end subroutine MatVec1
+
+  !> b = PSolve1(x) = -M^-1 x, M the vertical-line block of A (PreCond_Mod.f90:1555).
+  subroutine PSolve1(b, x)
+    type(PressureVelocity), dimension(:) :: b, x
+    integer(c_int) :: rc
+    rc = asam_sound_psolve(ctx, b(1)%p, x(1)%p)
+    call check(rc, 'asam_sound_psolve')
+  end subroutine PSolve1
+
+  subroutine AsamSoundFinalize()
+    integer(c_int) :: rc
+    if (c_associated(ctx)) then
+      rc = asam_sound_destroy(ctx)
+      ctx = c_null_ptr
+    end if
+  end subroutine AsamSoundFinalize
+
+  subroutine check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    AsamStatus = rc
+    if (rc /= 0) then
+      write(0, '(a,a,i0,a,a)') what, ' failed: ', rc, ' ', trim(asam_error_message(ctx))
+    end if
+  end subroutine check
+
+end module AsamSound_Mod
