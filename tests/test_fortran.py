@@ -16,6 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "asam_amd", "fortran")
 DRV = os.path.join(FDIR, "_build", "drv_solve_sound")
 FLANG = "/opt/rocm/bin/amdflang"
+LIBDIR = os.path.join(ROOT, "asam_amd", "lib")
 
 needs_flang = pytest.mark.skipif(not os.path.exists(FLANG), reason="amdflang not installed")
 
@@ -69,7 +70,8 @@ end program
         open(f, "w").write(src)
         exe = os.path.join(d, "s")
         subprocess.run([FLANG, "-I", os.path.join(FDIR, "_build"), f,
-                        os.path.join(FDIR, "_build", "asam_sound_iface.o"), "-o", exe],
+                        os.path.join(FDIR, "_build", "asam_sound_iface.o"),
+                        "-L" + LIBDIR, "-lasam_sound", "-Wl,-rpath," + LIBDIR, "-o", exe],
                        check=True, capture_output=True)
         vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
                                                check=True).stdout.split()]
