@@ -1,15 +1,6 @@
 // apply.hip -- stencil apply kernels (gfx950).
 //
-// Operator (one block, Neumann walls), per cell c with faces f to neighbours n:
-//   MultiTriT : (A x)_c = fa V_c x_c + sum_f k_f (DUT_c x_c - DUT_n x_n)
-//   MultiTriTB: row1 = fa V x1 + sum_f th_f k'_f (q_c - q_n) + vertical gravity * th_f
-//               row2 =    V x2 + sum_f      k'_f (q_c - q_n) + vertical gravity
-//               q = DUT x1; gravity flux of the z-face (k,k+1):
-//               B = g~ (V_k x2_k + V_k+1 x2_k+1), -B into cell k, +B into cell k+1.
-// This is the reference's nested tri-diagonal matrix (ComputeFull/Dual_1D..3D,
-// TriDiagT_Mod.f90:32-900; Compute_1D..3D, TriDiagTB_Mod.f90:32-780) written per face:
-// u = -k DUT_n, l = -k DUT_c, d += k DUT_c, and the 2x2 gravity entries
-// (TriDiagTB_Mod.f90:56-81) with g~ = Grav (beta0 dtP)^2 F / (V_k + V_k+1 + Eps).
+// The operator of one tile is march_tile (apply_tile.h).
 //
 // Kernel shapes: apply = one cell per thread over 64x4-column tiles of kp planes
 // (neighbour re-reads hit L1/L2; the tiles of an XCD are contiguous so its L2 holds
@@ -17,33 +8,9 @@
 // columns per wavefront (coalesced along x at every level), loads issued a window of
 // levels ahead of the recurrence, forward-sweep results staged in LDS for the
 // backward sweep.
-#include "kcommon.h"
+#include "apply_tile.h"
 
 namespace asam {
-
-// ===================================================================== apply
-// Fused options of one apply launch (compile-time, so no branch separates the loads
-// of a cell): residual form y = b - Op(x) (AF_B), copy of y (AF_Y2), x_eff = x + dx
-// written to xw (AF_DX), dots y.y (AF_YY) and y.u (AF_YU).
-enum { AF_B = 1, AF_Y2 = 2, AF_DX = 4, AF_YY = 8, AF_YU = 16 };
-
-template <typename CT, typename VX, typename VO, typename VU>
-struct KApply {
-  Coef<CT> c;
-  int nx, ny, nz;
-  int gx, gy, kp, ntiles;  // tiles: bx x by columns x kp planes, ntiles in logical order
-  int bxw, byw, xcd;
-  CT fa;
-  const VX* x;
-  const float* dx;
-  double* xw;
-  const double* b;
-  VO* y;
-  VO* y2;
-  const VU* u;
-  double* partial;
-  const int* done;
-};
 
 struct ApplyShape {
   int gx, gy, kp, ntiles, nblocks;
@@ -232,176 +199,15 @@ __global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
   }
 }
 
-// z-march variant: a thread owns one column of a tile and walks its kp planes upwards,
-// carrying the cells below/at/above and the z-face coefficients in registers, so each
-// level loads only the next cell up and the four horizontal neighbours.
+// z-march variant: one tile per workgroup (tiles of an XCD contiguous, see k_apply).
 template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typename VU>
 __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
   if (a.done && *a.done) return;
-  const int nx = a.nx, ny = a.ny, nz = a.nz;
-  const long sy = nx, sz = (long)nx * ny;
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
   const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
   double acc[2] = {0.0, 0.0};
-  const CT* __restrict__ kx = a.c.kx;
-  const CT* __restrict__ ky = a.c.ky;
-  const CT* __restrict__ kz = a.c.kz;
-  const CT* __restrict__ tx = a.c.tx;
-  const CT* __restrict__ ty = a.c.ty;
-  const CT* __restrict__ tz = a.c.tz;
-  const CT* __restrict__ gzz = a.c.gz;
-  const CT* __restrict__ vol = a.c.vol;
-  const CT* __restrict__ dut = a.c.dut;
-  const VX* __restrict__ X = a.x;
-  const float* __restrict__ DX = a.dx;
-  const CT fa = a.fa;
-  if (L < a.ntiles) {
-    const int bx = L % a.gx, rest = L / a.gx, by = rest % a.gy, kt = rest / a.gy;
-    const int i = bx * a.bxw + threadIdx.x, j = by * a.byw + threadIdx.y;
-    const int k0 = kt * a.kp, k1 = min(nz, k0 + a.kp);
-    if (i < nx && j < ny) {
-      const long cb = i + sy * j;
-      const long dW = i > 0 ? -1 : 0, dE = i + 1 < nx ? 1 : 0;
-      const long dS = j > 0 ? -sy : 0, dN = j + 1 < ny ? sy : 0;
-      if constexpr (NP == 1) {
-        auto LX = [&](long m) -> CT {
-          CT v = (CT)X[m];
-          if constexpr ((FL & AF_DX) != 0) v += (CT)DX[m];
-          return v;
-        };
-        long c = cb + sz * k0;
-        CT xm = 0, dm = 0, kzm = 0;
-        if (k0 > 0) { xm = LX(c - sz); dm = dut[c - sz]; kzm = kz[c - sz]; }
-        CT x0 = LX(c), d0 = dut[c];
-        for (int k = k0; k < k1; ++k, c += sz) {
-          const long mU = k + 1 < nz ? c + sz : c;
-          const CT xp = LX(mU), dp = dut[mU], kz0 = kz[c];
-          const long mW = c + dW, mE = c + dE, mS = c + dS, mN = c + dN;
-          CT s = fa * vol[c] * x0;
-          if constexpr (TR == 0) {
-            const CT q0 = d0 * x0;
-            s += kx[mW] * (q0 - dut[mW] * LX(mW));
-            s += kx[c] * (q0 - dut[mE] * LX(mE));
-            s += ky[mS] * (q0 - dut[mS] * LX(mS));
-            s += ky[c] * (q0 - dut[mN] * LX(mN));
-            s += kzm * (q0 - dm * xm);
-            s += kz0 * (q0 - dp * xp);
-          } else {
-            CT h = kx[mW] * (x0 - LX(mW));
-            h += kx[c] * (x0 - LX(mE));
-            h += ky[mS] * (x0 - LX(mS));
-            h += ky[c] * (x0 - LX(mN));
-            h += kzm * (x0 - xm);
-            h += kz0 * (x0 - xp);
-            s += d0 * h;
-          }
-          double yv;
-          if constexpr ((FL & AF_B) != 0) {
-            const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 ==
-                                                     (CT)0;
-            yv = dead ? 0.0 : a.b[c] + (double)s;
-          } else {
-            yv = -(double)s;
-          }
-          if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
-          a.y[c] = (VO)yv;
-          if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv;
-          if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
-          if constexpr ((FL & AF_YU) != 0) acc[1] += yv * (double)a.u[c];
-          xm = x0; dm = d0; kzm = kz0; x0 = xp; d0 = dp;
-        }
-      } else {
-        auto LX = [&](long m) -> V2<CT> {
-          V2<VX> v = ld2(X, m);
-          V2<CT> r{(CT)v.a, (CT)v.b};
-          if constexpr ((FL & AF_DX) != 0) {
-            V2<float> e = ld2(DX, m);
-            r.a += (CT)e.a; r.b += (CT)e.b;
-          }
-          return r;
-        };
-        long c = cb + sz * k0;
-        V2<CT> xm{0, 0};
-        CT dm = 0, vm = 0, kzm = 0, tzm = 0, gzm = 0;
-        if (k0 > 0) {
-          const long m = c - sz;
-          xm = LX(m); dm = dut[m]; vm = vol[m]; kzm = kz[m]; tzm = tz[m]; gzm = gzz[m];
-        }
-        V2<CT> x0 = LX(c);
-        CT d0 = dut[c], v0 = vol[c];
-        for (int k = k0; k < k1; ++k, c += sz) {
-          const long mU = k + 1 < nz ? c + sz : c;
-          const V2<CT> xp = LX(mU);
-          const CT dp = dut[mU], vp = vol[mU];
-          const CT kz0 = kz[c], tz0 = tz[c], gz0 = gzz[c];
-          const long mW = c + dW, mE = c + dE, mS = c + dS, mN = c + dN;
-          const V2<CT> xW = LX(mW), xE = LX(mE), xS = LX(mS), xN = LX(mN);
-          CT s1 = fa * v0 * x0.a, s2 = v0 * x0.b;
-          if constexpr (TR == 0) {
-            const CT q0 = d0 * x0.a;
-            auto hf = [&](CT kk, CT th, CT dn, CT xn) {
-              const CT t = kk * (q0 - dn * xn);
-              s1 += th * t; s2 += t;
-            };
-            hf(kx[mW], tx[mW], dut[mW], xW.a);
-            hf(kx[c], tx[c], dut[mE], xE.a);
-            hf(ky[mS], ty[mS], dut[mS], xS.a);
-            hf(ky[c], ty[c], dut[mN], xN.a);
-            {
-              const CT t = kzm * (q0 - dm * xm.a);
-              const CT B = gzm * (vm * xm.b + v0 * x0.b);
-              s1 += tzm * (t + B); s2 += t + B;
-            }
-            {
-              const CT t = kz0 * (q0 - dp * xp.a);
-              const CT B = gz0 * (v0 * x0.b + vp * xp.b);
-              s1 += tz0 * (t - B); s2 += t - B;
-            }
-          } else {
-            CT h = 0;
-            auto hf = [&](CT kk, CT th, V2<CT> xn) {
-              h += kk * ((th * x0.a + x0.b) - (th * xn.a + xn.b));
-            };
-            hf(kx[mW], tx[mW], xW);
-            hf(kx[c], tx[c], xE);
-            hf(ky[mS], ty[mS], xS);
-            hf(ky[c], ty[c], xN);
-            {
-              const CT zc = tzm * x0.a + x0.b, zn = tzm * xm.a + xm.b;
-              h += kzm * (zc - zn);
-              s2 += gzm * v0 * (zc - zn);
-            }
-            {
-              const CT zc = tz0 * x0.a + x0.b, zn = tz0 * xp.a + xp.b;
-              h += kz0 * (zc - zn);
-              s2 += gz0 * v0 * (zn - zc);
-            }
-            s1 += d0 * h;
-          }
-          double yv1, yv2;
-          if constexpr ((FL & AF_B) != 0) {
-            const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
-            const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 == (CT)0;
-            yv1 = dead ? 0.0 : bb.x + (double)s1;
-            yv2 = dead ? 0.0 : bb.y + (double)s2;
-          } else {
-            yv1 = -(double)s1; yv2 = -(double)s2;
-          }
-          if constexpr ((FL & AF_DX) != 0) st2(a.xw, c, (double)x0.a, (double)x0.b);
-          st2(a.y, c, yv1, yv2);
-          if constexpr ((FL & AF_Y2) != 0) st2(a.y2, c, yv1, yv2);
-          if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
-          if constexpr ((FL & AF_YU) != 0) {
-            const V2<VU> uu = ld2(a.u, c);
-            acc[1] += yv1 * (double)uu.a + yv2 * (double)uu.b;
-          }
-          xm = x0; dm = d0; vm = v0; kzm = kz0; tzm = tz0; gzm = gz0;
-          x0 = xp; d0 = dp; v0 = vp;
-        }
-      }
-    }
-  }
+  march_tile<NP, TR, FL, CT, VX, VO, VU>(a, L, acc);
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
   }

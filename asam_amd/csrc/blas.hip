@@ -7,8 +7,7 @@
 // dot product is bitwise reproducible run to run.
 #include <math.h>
 
-#include "fin.h"
-#include "internal.h"
+#include "kcommon.h"
 
 namespace asam {
 
@@ -61,27 +60,8 @@ __global__ __launch_bounds__(256) void k_bicg_update(long n, VT* __restrict__ x,
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride)
       x[e] = (VT)((double)x[e] + alpha * (double)ph[e]);
   } else {
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-      double xv = (double)x[e];
-      if constexpr (sizeof(VT) == 8) {
-        xv = xv + alpha * (double)ph[e];
-        xv = xv + omega * (double)rh[e];
-        double rv = (double)r[e] - omega * (double)t[e];
-        x[e] = (VT)xv;
-        r[e] = (VT)rv;
-        acc[0] += rv * rv;
-        acc[1] += (double)rt[e] * rv;
-      } else {
-        float xf = (float)x[e];
-        xf = xf + (float)alpha * (float)ph[e];
-        xf = xf + (float)omega * (float)rh[e];
-        float rf = (float)r[e] - (float)omega * (float)t[e];
-        x[e] = (VT)xf;
-        r[e] = (VT)rf;
-        acc[0] += (double)rf * (double)rf;
-        acc[1] += (double)rt[e] * (double)rf;
-      }
-    }
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride)
+      bicg_upd_elem<VT>(e, x, ph, rh, r, t, rt, alpha, omega, acc);
   }
   block_reduce_store_b<2>(acc, partial, blockIdx.x);
 }

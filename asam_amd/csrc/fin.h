@@ -137,9 +137,14 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       // FP64 residual of the refinement step; sets up the FP32 inner BiCGStab on A d = r.
       if (st->outer_done) break;
       if (st->outer > 0) {  // inner iterations of the previous refinement step
-        int res = st->result;
-        st->inner_total += res < 0 ? -res : (st->status == 1 ? res / 2 : res);
+        int res = st->result, it;
+        if (st->done) it = res < 0 ? -res : (st->status == 1 ? res / 2 : res);
+        else it = st->iter - 1;  // inner solve cut short by a fixed schedule
+        st->inner_total += it;
+        if (st->outer <= 16) st->hist[st->outer - 1] = it;
       }
+      // x after this step's update x += d is in buffer (outer & 1) of the pair
+      st->xpar = st->outer & 1;
       double r = sqrt(S[0]);
       if (st->outer == 0) st->outer_r0 = r;
       st->outer_r = r;

@@ -79,6 +79,9 @@ struct KState {
   // mixed refinement
   double outer_r0, outer_r, outer_tol, inner_tol;
   int outer, inner_total, outer_done, pad2;
+  int xpar;             // ensemble refinement: which of the two x buffers holds x
+  int pad3;
+  int hist[16];         // inner iterations of refinement step q (q < 16)
 };
 
 // ------------------------------------------------------------------ set-up

@@ -61,12 +61,18 @@ class AsamError(RuntimeError):
 _lib = None
 
 
-def exported_names_from_header(path: str = HEADER):
-    """Function names declared in include/asam_sound.h."""
+def exported_names_from_header(path: str = None):
+    """Function names declared in include/asam_sound.h and include/asam_ensemble.h (or
+    in the one header named)."""
     import re
-    txt = open(path).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\**\s*(asam_\w+)\s*\(", txt,
-                                 flags=re.M)))
+    paths = [path] if path else [HEADER, os.path.join(os.path.dirname(HEADER),
+                                                      "asam_ensemble.h")]
+    names = set()
+    for pth in paths:
+        txt = open(pth).read()
+        names |= set(re.findall(r"^\s*(?:int|long|const char \*)\s*\**\s*(asam_\w+)\s*\(",
+                                txt, flags=re.M))
+    return sorted(names)
 
 
 def load(path: str = LIB_PATH):

@@ -1,4 +1,5 @@
-"""The C-ABI library loads and exports every entry point include/asam_sound.h declares,
+"""The C-ABI library loads and exports every entry point include/asam_sound.h and
+include/asam_ensemble.h declare,
 and the ctypes mirrors of the header structs have the C layout (no GPU calls)."""
 import os
 import subprocess
@@ -12,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol(sound_mod):
     L = sound_mod.load()
     names = sound_mod.exported_names_from_header()
-    assert len(names) >= 30
+    assert len(names) >= 45 and "asam_ens_solve_device" in names
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     out = subprocess.run(["nm", "-D", "--defined-only", sound_mod.LIB_PATH],
@@ -25,11 +26,12 @@ def test_struct_layouts_match_header(sound_mod):
     src = r'''
 #include <stdio.h>
 #include <stddef.h>
-#include "asam_sound.h"
+#include "asam_ensemble.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(asam_sound_grid), sizeof(asam_sound_control),
-         sizeof(asam_sound_info), sizeof(asam_sound_tuning),
-         offsetof(asam_sound_grid, beta0_dt), offsetof(asam_sound_info, ms));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(asam_sound_grid),
+         sizeof(asam_sound_control), sizeof(asam_sound_info), sizeof(asam_sound_tuning),
+         offsetof(asam_sound_grid, beta0_dt), offsetof(asam_sound_info, ms),
+         sizeof(asam_ens_tuning), sizeof(asam_ens_stats));
   return 0;
 }
 '''
@@ -42,15 +44,19 @@ int main(void) {
         vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
                                                check=True).stdout.split()]
     S = sound_mod
+    from asam_amd import ensemble as E
     assert vals == [C.sizeof(S.Grid), C.sizeof(S.Control), C.sizeof(S.Info),
-                    C.sizeof(S.Tuning), S.Grid.beta0_dt.offset, S.Info.ms.offset]
+                    C.sizeof(S.Tuning), S.Grid.beta0_dt.offset, S.Info.ms.offset,
+                    C.sizeof(E.EnsTuning), C.sizeof(E.EnsStats)]
 
 
 def test_no_cpu_fallback_in_product(sound_mod):
     """The product path never imports the oracle: the host mirror fails loudly without
     the HIP library."""
-    src = open(os.path.join(ROOT, "asam_amd", "sound.py")).read()
-    assert "oracle" not in src
+    for f in ("sound.py", "ensemble.py", "rosenbrock.py", "workload.py"):
+        src = open(os.path.join(ROOT, "asam_amd", f)).read()
+        assert "import oracle" not in src and "from oracle" not in src, f
+        assert "pyoracle" not in src and "liborc" not in src, f
     with pytest.raises(ImportError):
         sound_mod.load.__wrapped__ if hasattr(sound_mod.load, "__wrapped__") else None
         import importlib
