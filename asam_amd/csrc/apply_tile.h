@@ -60,6 +60,13 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
   const CT* __restrict__ dut = a.c.dut;
   const VX* __restrict__ X = a.x;
   const float* __restrict__ DX = a.dx;
+  // Outputs and the dot operand through restrict copies: the compiler may then issue the
+  // loads of a level (u and b included) before the stores of the previous one.
+  VO* __restrict__ Y = a.y;
+  VO* __restrict__ Y2 = a.y2;
+  const VU* __restrict__ U = a.u;
+  double* __restrict__ XW = a.xw;
+  const double* __restrict__ Bv = a.b;
   const CT fa = a.fa;
   if (L < a.ntiles) {
     const int bx = L % a.gx, rest = L / a.gx, by = rest % a.gy, kt = rest / a.gy;
@@ -83,6 +90,10 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
           const long mU = k + 1 < nz ? c + sz : c;
           const CT xp = LX(mU), dp = dut[mU], kz0 = kz[c];
           const long mW = c + dW, mE = c + dE, mS = c + dS, mN = c + dN;
+          VU u0 = 0;
+          if constexpr ((FL & AF_YU) != 0) u0 = U[c];
+          double b0 = 0.0;
+          if constexpr ((FL & AF_B) != 0) b0 = Bv[c];
           CT s = fa * vol[c] * x0;
           if constexpr (TR == 0) {
             const CT q0 = d0 * x0;
@@ -105,15 +116,15 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
           if constexpr ((FL & AF_B) != 0) {
             const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 ==
                                                      (CT)0;
-            yv = dead ? 0.0 : a.b[c] + (double)s;
+            yv = dead ? 0.0 : b0 + (double)s;
           } else {
             yv = -(double)s;
           }
-          if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
-          a.y[c] = (VO)yv;
-          if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv;
+          if constexpr ((FL & AF_DX) != 0) XW[c] = (double)x0;
+          Y[c] = (VO)yv;
+          if constexpr ((FL & AF_Y2) != 0) Y2[c] = (VO)yv;
           if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
-          if constexpr ((FL & AF_YU) != 0) acc[1] += yv * (double)a.u[c];
+          if constexpr ((FL & AF_YU) != 0) acc[1] += yv * (double)u0;
           xm = x0; dm = d0; kzm = kz0; x0 = xp; d0 = dp;
         }
       } else {
@@ -142,6 +153,10 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
           const CT kz0 = kz[c], tz0 = tz[c], gz0 = gzz[c];
           const long mW = c + dW, mE = c + dE, mS = c + dS, mN = c + dN;
           const V2<CT> xW = LX(mW), xE = LX(mE), xS = LX(mS), xN = LX(mN);
+          V2<VU> u0{0, 0};
+          if constexpr ((FL & AF_YU) != 0) u0 = ld2(U, c);
+          double2 b0 = make_double2(0.0, 0.0);
+          if constexpr ((FL & AF_B) != 0) b0 = *reinterpret_cast<const double2*>(Bv + 2 * c);
           CT s1 = fa * v0 * x0.a, s2 = v0 * x0.b;
           if constexpr (TR == 0) {
             const CT q0 = d0 * x0.a;
@@ -186,21 +201,17 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
           }
           double yv1, yv2;
           if constexpr ((FL & AF_B) != 0) {
-            const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
             const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz0 == (CT)0;
-            yv1 = dead ? 0.0 : bb.x + (double)s1;
-            yv2 = dead ? 0.0 : bb.y + (double)s2;
+            yv1 = dead ? 0.0 : b0.x + (double)s1;
+            yv2 = dead ? 0.0 : b0.y + (double)s2;
           } else {
             yv1 = -(double)s1; yv2 = -(double)s2;
           }
-          if constexpr ((FL & AF_DX) != 0) st2(a.xw, c, (double)x0.a, (double)x0.b);
-          st2(a.y, c, yv1, yv2);
-          if constexpr ((FL & AF_Y2) != 0) st2(a.y2, c, yv1, yv2);
+          if constexpr ((FL & AF_DX) != 0) st2(XW, c, (double)x0.a, (double)x0.b);
+          st2(Y, c, yv1, yv2);
+          if constexpr ((FL & AF_Y2) != 0) st2(Y2, c, yv1, yv2);
           if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
-          if constexpr ((FL & AF_YU) != 0) {
-            const V2<VU> uu = ld2(a.u, c);
-            acc[1] += yv1 * (double)uu.a + yv2 * (double)uu.b;
-          }
+          if constexpr ((FL & AF_YU) != 0) acc[1] += yv1 * (double)u0.a + yv2 * (double)u0.b;
           xm = x0; dm = d0; vm = v0; kzm = kz0; tzm = tz0; gzm = gz0;
           x0 = xp; d0 = dp; v0 = vp;
         }

@@ -971,6 +971,13 @@ extern "C" int asam_ens_dev_download(asam_ens_ctx* c, void* dst, const void* src
   return 0;
 }
 
+extern "C" int asam_ens_dev_memset(asam_ens_ctx* c, void* dst, int value, size_t bytes) {
+  if (!c || !dst) return ASAM_E_INVALID;
+  EHIP(c, hipMemsetAsync(dst, value, bytes, c->stream));
+  EHIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 // ------------------------------------------------------------------ autotuner
 namespace {
 

@@ -62,6 +62,7 @@ def load():
         "asam_ens_dev_free": (i, [_vp, _vp]),
         "asam_ens_dev_upload": (i, [_vp, _vp, _vp, C.c_size_t]),
         "asam_ens_dev_download": (i, [_vp, _vp, _vp, C.c_size_t]),
+        "asam_ens_dev_memset": (i, [_vp, _vp, i, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -93,6 +94,10 @@ class EnsBuffer:
         self.ens._chk(self.ens.L.asam_ens_dev_download(self.ens.h, out.ctypes.data_as(_vp),
                                                        self.ptr, self.nbytes))
         return out
+
+    def zero(self):
+        self.ens._chk(self.ens.L.asam_ens_dev_memset(self.ens.h, self.ptr, 0, self.nbytes))
+        return self
 
     def free(self):
         if self.ptr and getattr(self.ens, "h", None):

@@ -67,7 +67,11 @@ def face_weights(prob, theta):
 
 
 class LinearRosenbrock:
-    """Ros3P steps u -> u' of the linear stage model on one solver context."""
+    """Ros3P steps u -> u' of the linear stage model on one solver context.
+
+    The solver supplies device vectors and the operations used here: vec, alloc, zero,
+    axpby, mass (V x), apply_device (MatVec1 or its transpose), solve_device (SolveSound
+    or the transposed solve) and theta_vjp; SoundSolver is the GPU implementation."""
 
     def __init__(self, solver, dt, tol=1e-13, maxiter=500):
         self.s = solver
@@ -82,7 +86,7 @@ class LinearRosenbrock:
 
     # -- device helpers ------------------------------------------------------------
     def _zero(self, b):
-        self.s._chk(self.s.L.asam_dev_memset(self.s.h, b.ptr, 0, b.nbytes))
+        self.s.zero(b)
 
     def _copy(self, dst, src):
         self._zero(dst)

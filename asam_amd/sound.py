@@ -307,6 +307,9 @@ class SoundSolver:
             buf.upload(_c(a))
         return buf
 
+    def zero(self, buf: DeviceBuffer):
+        self._chk(self.L.asam_dev_memset(self.h, buf.ptr, 0, buf.nbytes))
+
     def apply_device(self, y: DeviceBuffer, x: DeviceBuffer, transpose=False):
         self._chk(self.L.asam_sound_apply_device(self.h, y.ptr, x.ptr, int(transpose)))
 

@@ -44,6 +44,11 @@ def parse():
                     help="time the copy, apply and line kernels in isolation on config CFG "
                          "(2: 256x256x64, 3: 512x512x96) and exit (profiler runs)")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="M",
+                    help="measure the batched ensemble solve of M uneven members (64: the "
+                         "64-member ensemble, nx,ny~U{64..256}, nz~U{40..64}) and exit")
+    ap.add_argument("--size-scale", type=float, default=1.0,
+                    help="ensemble: horizontal size factor of the members (1: full size)")
     return ap.parse_args()
 
 
@@ -103,16 +108,24 @@ def roofline_from_profile(prof, np1):
             "bytes_per_launch": int(bytes_per_launch)}, name
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE (scaled by
-    the copy-kernel factor), recorded in asam_amd/traffic.json; None if not measured."""
+def read_traffic(kernel, workload):
+    """HBM bytes per launch of a kernel class measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (each scaled by what it reads for a copy of known size), recorded in
+    asam_amd/traffic.json by asam_amd/traffic.py; None unless measured on this workload."""
     path = os.path.join(ROOT, "asam_amd", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t.get(kernel)
-    except Exception:
+    except (OSError, ValueError):
         return None
+    if t.get("_about", {}).get("workload") != workload:
+        return None
+    e = t.get(kernel)
+    return e["bytes_per_launch"] if e else None
+
+
+def traffic_workload(np1, tol):
+    return f"bench.py default solve: 256x256x64 cut-cell, np1={np1}, tol {tol:g}"
 
 
 def cpu_baseline(p, tol, seconds):
@@ -156,6 +169,120 @@ def hbm_roofline(np1):
     return out
 
 
+def ensemble_bench(a):
+    """Batched ensemble: all members solved together (FP32 inner / FP64 refinement to tol,
+    one HIP-graph replay per solve, autotuned tiles) vs the same members solved one after
+    the other by the plain FP64 drop-in (and by the single-domain mixed solve)."""
+    from asam_amd import ensemble, sound, workload
+    t0 = time.perf_counter()
+    ps = workload.ensemble_members(a.ensemble, seed=5, np1=a.np1, size_scale=a.size_scale)
+    cells = sum(p.grid.ncell for p in ps)
+    print(f"[ensemble] {len(ps)} members, {cells / 1e6:.1f} M cells, generated in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    E = ensemble.Ensemble(ps)
+    tune, tms = E.autotune(reps=3)
+    print(f"[ensemble] autotuned {tune}", file=sys.stderr, flush=True)
+    b = E.concat([p.b for p in ps])
+    bbuf, xbuf = E.buffer(b), E.buffer()
+    for _ in range(a.warmup):
+        xbuf.zero()
+        E.solve_device(xbuf, bbuf, a.tol)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        xbuf.zero()
+        infos = E.solve_device(xbuf, bbuf, a.tol)
+    t_ens = (time.perf_counter() - t0) / a.steps
+    st = E.stats()
+    xe = E.split(xbuf.download())
+    E.close()
+    del bbuf, xbuf
+    print(f"[ensemble] batched solve {t_ens * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    # per-launch algorithmic bytes of the tuned kernels (all members active)
+    np1 = a.np1
+    coefw = 5 if np1 == 1 else 9
+    bytes_cell = {"apply_fp32": coefw * 4 + np1 * 12, "line_fp32": 3 * np1 * np1 * 4 + np1 * 16,
+                  "update_fp32": 8 * np1 * 4}
+    kern = {}
+    for (k, bc), ms in zip(bytes_cell.items(), tms):
+        gbs = bc * cells / (ms * 1e-3) / 1e9
+        kern[k] = {"avg_launch_us": round(ms * 1e3, 1), "bytes_per_cell": bc,
+                   "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    # sequential baselines: one drop-in context per member, solves timed on resident inputs
+    t64 = tmx = 0.0
+    worst_res = worst_diff = 0.0
+    its64 = []
+    for m, p in enumerate(ps):
+        s = sound.SoundSolver.from_problem(p)
+        xb, bb = s.vec(), s.vec(p.b)
+        for prec in ("FP64", "MIXED"):
+            s.set_control(precision=prec)
+            s._chk(s.L.asam_dev_memset(s.h, xb.ptr, 0, s.n * 8))
+            s.solve_device(xb, bb, 500, a.tol)  # warm-up (kernels, iteration history)
+            reps = max(1, a.steps // 4)
+            tt = time.perf_counter()
+            for _ in range(reps):
+                s._chk(s.L.asam_dev_memset(s.h, xb.ptr, 0, s.n * 8))
+                _, _, inf = s.solve_device(xb, bb, 500, a.tol)
+            dt = (time.perf_counter() - tt) / reps
+            if prec == "FP64":
+                t64 += dt
+                its64.append(inf.iterations)
+                x64 = xb.download()
+            else:
+                tmx += dt
+        r = p.b - s.MatVec1(xe[m])
+        worst_res = max(worst_res, float(np.linalg.norm(r) / np.linalg.norm(p.b)))
+        worst_diff = max(worst_diff, float(np.linalg.norm(xe[m] - x64) / np.linalg.norm(x64)))
+        s.close()
+    line = {
+        "metric": "ensemble members solved/s (FP32-inner + FP64-refined BiCGStab to 1e-8, "
+                  "all members in one HIP-graph replay)",
+        "value": round(len(ps) / t_ens, 2),
+        "unit": "members/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(t_ens * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 residual, dots, refinement",
+        "data": "synthetic (seeded members: Agnesi ridge H~U(0,1.5) km, bubble ~U(1,3) K, "
+                "b~N(0,1))",
+        "config": {"workload": f"{len(ps)}-member uneven ensemble, nx,ny~U{{64..256}}x"
+                               f"{a.size_scale:g}, nz~U{{40..64}}, "
+                               f"{'MultiTriTB' if np1 == 2 else 'MultiTriT'}",
+                   "members": len(ps), "cells": cells, "unknowns": cells * np1},
+        "graph": st,
+        "autotuned": tune,
+        "kernels": kern,
+        "sequential_fp64": {"members_per_s": round(len(ps) / t64, 2),
+                            "ms_total": round(t64 * 1e3, 3),
+                            "iterations_min_max": [min(its64), max(its64)]},
+        "sequential_mixed_single_domain": {"members_per_s": round(len(ps) / tmx, 2),
+                                           "ms_total": round(tmx * 1e3, 3)},
+        "speedup_vs_sequential_fp64": round(t64 / t_ens, 3),
+        "max_true_rel_residual": worst_res,
+        "max_rel_diff_vs_fp64_gpu": worst_diff,
+        "inner_iterations_min_max": [min(i.iterations for i in infos),
+                                     max(i.iterations for i in infos)],
+    }
+    if not a.no_cpu:
+        from oracle import pyoracle
+        threads = min(os.cpu_count() or 1, 16)
+        pyoracle.set_threads(threads)
+        tt, done = 0.0, 0
+        for p in ps:
+            orc = pyoracle.Oracle(p)
+            t1 = time.perf_counter()
+            orc.bicgstab(p.b, 500, a.tol)
+            tt += time.perf_counter() - t1
+            done += 1
+            if tt > a.cpu_seconds:
+                break
+        line["cpu_baseline"] = {"value": round(done / tt, 4), "unit": "members/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"first {done} members solved by the FP64 C "
+                                          f"restatement (OpenMP over columns), {tt:.1f} s"}
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -166,6 +293,9 @@ def main():
         sys.exit(2)
     if a.kernel_probe:
         kernel_probe(a.kernel_probe, a.np1, a.reps)
+        return
+    if a.ensemble:
+        ensemble_bench(a)
         return
     from asam_amd import sound, workload
     p = workload.config(2, np1=a.np1)
@@ -195,7 +325,7 @@ def main():
 
     roof, dom = roofline_from_profile(profmx, a.np1)
     if roof is not None:
-        roof["traffic"] = read_traffic(dom)
+        roof["traffic"] = read_traffic(dom, traffic_workload(a.np1, a.tol))
     line = {
         "metric": "Krylov solves/s (FP32-inner + FP64-refined BiCGStab to 1e-8), 256x256x64 cut-cell",
         "value": round(1.0 / tmx, 3),

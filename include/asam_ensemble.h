@@ -97,6 +97,7 @@ int asam_ens_dev_alloc(asam_ens_ctx *ctx, void **ptr, size_t bytes);
 int asam_ens_dev_free(asam_ens_ctx *ctx, void *ptr);
 int asam_ens_dev_upload(asam_ens_ctx *ctx, void *dst, const void *src, size_t bytes);
 int asam_ens_dev_download(asam_ens_ctx *ctx, void *dst, const void *src, size_t bytes);
+int asam_ens_dev_memset(asam_ens_ctx *ctx, void *dst, int value, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -236,3 +236,46 @@ def test_bicgstab_reference_return_conventions(oracle_mod):
     r0 = np.linalg.norm(p.b)
     assert st == 0 and tl <= 1e-8 * r0 and abs(mi) >= 1
     assert np.isclose(np.linalg.norm(p.b - o.matvec1(x)), tl, rtol=1e-6)
+
+
+def test_weights_match_equation_of_state(oracle_mod):
+    """DUT is dp/d(rho theta) of p = p0 (Rd rho theta / p0)^(Cpd/Cvd) (central differences
+    on the seeded bubble state), and the face theta is exact for a uniform theta whatever
+    the cut volumes, the midpoint for a theta linear in x or z on equal volumes, and the
+    adjacent cell's value on wall faces."""
+    p = tiny(1)
+    o = oracle_mod.Oracle(p)
+    DTU, DTV, DTW, DUT = o.weights()
+    act = p.VolC > 0
+    gam = W.CPD / W.CVD
+
+    def eos(rt):
+        return W.P0 * (W.RD * rt / W.P0) ** gam
+    rt = p.rhotheta
+    h = 1e-5 * rt
+    fd = (eos(rt + h) - eos(rt - h)) / (2 * h)
+    assert np.allclose(DUT[act], fd[act], rtol=1e-8)
+    # uniform theta on the cut grid: every face with a live neighbour carries theta
+    q = tiny(1)
+    q.rhotheta = q.rho * 300.0
+    DTU, DTV, DTW, _ = oracle_mod.Oracle(q).weights()
+    V = q.VolC
+    live_x = np.zeros(DTU.shape, bool)
+    live_x[:, :, 1:-1] = (V[:, :, 1:] + V[:, :, :-1]) > 0
+    live_x[:, :, [0, -1]] = True
+    assert np.allclose(DTU[live_x], 300.0, rtol=1e-12)
+    live_z = np.zeros(DTW.shape, bool)
+    live_z[1:-1] = (V[1:] + V[:-1]) > 0
+    live_z[[0, -1]] = True
+    assert np.allclose(DTW[live_z], 300.0, rtol=1e-12)
+    # theta linear in x and in z on a flat grid: faces sit at the midpoints
+    f = W.make_problem(7, 4, 6, 1000.0, 1000.0, 250.0, H=0.0, np1=1, seed=2)
+    kk, jj, ii = np.meshgrid(np.arange(6), np.arange(4), np.arange(7), indexing="ij")
+    th = 300.0 + 0.5 * ii + 0.25 * kk
+    f.rhotheta = f.rho * th
+    DTU, DTV, DTW, _ = oracle_mod.Oracle(f).weights()
+    assert np.allclose(DTU[:, :, 1:-1], 0.5 * (th[:, :, 1:] + th[:, :, :-1]), rtol=1e-13)
+    assert np.allclose(DTW[1:-1], 0.5 * (th[1:] + th[:-1]), rtol=1e-13)
+    assert np.allclose(DTV[:, 1:-1, :], 0.5 * (th[:, 1:, :] + th[:, :-1, :]), rtol=1e-13)
+    assert np.allclose(DTU[:, :, 0], th[:, :, 0]) and np.allclose(DTU[:, :, -1], th[:, :, -1])
+    assert np.allclose(DTW[0], th[0]) and np.allclose(DTW[-1], th[-1])
