@@ -271,10 +271,7 @@ __global__ __launch_bounds__(256) void k_ens_finalize(int kind, int nq, int srcs
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-  }
+  for (int q = 0; q < 4; ++q) v[q] = wave_sum(v[q]);
   if ((tid & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) sh[q][tid >> 6] = v[q];

@@ -1,8 +1,8 @@
 // blas.hip -- fused vector updates, deterministic two-stage reductions and the
 // single-thread Krylov control tails (scalars never leave the device).
 //
-// Reduction stage 1 lives in the producing kernel (wavefront butterfly via
-// __shfl_xor, then LDS across waves, one partial per workgroup); stage 2 is a
+// Reduction stage 1 lives in the producing kernel (wavefront sum in registers via DPP,
+// then LDS across waves, one partial per workgroup); stage 2 is a
 // one-workgroup finalize kernel that sums the partials in a fixed order, so a
 // dot product is bitwise reproducible run to run.
 #include <math.h>
@@ -16,10 +16,7 @@ __device__ __forceinline__ void block_reduce_store_b(double (&v)[NQ], double* pa
   __shared__ double sh[NQ][16];
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-  }
+  for (int q = 0; q < NQ; ++q) v[q] = wave_sum(v[q]);
   if ((tid & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) sh[q][tid >> 6] = v[q];
@@ -243,10 +240,7 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, int nq, int nblk, i
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-  }
+  for (int q = 0; q < 4; ++q) v[q] = wave_sum(v[q]);
   if ((tid & 63) == 0) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) sh[q][tid >> 6] = v[q];

@@ -8,16 +8,39 @@
 namespace asam {
 
 // ===================================================================== helpers
+// Sum over the 64 lanes of a wavefront in registers (DPP): swaps of lanes 1 and 2 apart,
+// the half-row mirror and the row mirror leave every lane of a 16-lane row holding the row
+// total; the four row totals are then read out (readlane) and added in a fixed order, so
+// the result is the same in every lane and bitwise reproducible.  All 64 lanes must be
+// active at the call.
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return v + __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double row_lane(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]: lane ^ 1
+  v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]: lane ^ 2
+  v = dpp_add<0x141>(v);  // row_half_mirror: the two quads of each 8-lane half
+  v = dpp_add<0x140>(v);  // row_mirror: the two halves of each 16-lane row
+  return (row_lane(v, 0) + row_lane(v, 16)) + (row_lane(v, 32) + row_lane(v, 48));
+}
+
 template <int NQ>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NQ], double* partial, int blk,
                                                    int tid, int nthreads,
                                                    long stride = kRedBlocks) {
   __shared__ double sh[NQ][16];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_xor(v[q], off, 64);
-  }
+  for (int q = 0; q < NQ; ++q) v[q] = wave_sum(v[q]);
   const int lane = tid & 63, wid = tid >> 6;
   if (lane == 0) {
 #pragma unroll

@@ -194,3 +194,23 @@ def test_config2_full_size_parity(np1, oracle_mod, sound_mod):
     print(f"cfg2 np1={np1} mixed: inner {im.iterations} outer {im.restarts} true rel res "
           f"{rr:.3e} err vs FP64 oracle {em:.3e}, {im.ms:.2f} ms")
     assert rr <= 1.0001e-8
+
+
+def test_device_dot_matches_fsum(sound_mod):
+    """Two-stage reduction (DPP wavefront sums, LDS across waves, fixed-order finalize)
+    against a correctly rounded sum, within the FP64 bound n u sum|x_i y_i|, on lengths
+    that leave partial wavefronts and workgroups; repeated calls agree bit for bit."""
+    import math
+    p = W.config(4, np1=1, scale=0.25)
+    s = sound_mod.SoundSolver.from_problem(p)
+    for n in (1, 63, 64, 65, 1000, 12345, (1 << 20) + 3):
+        n = min(n, s.n)
+        x = W.normal(s.n, 60 + n)
+        y = W.uniform01(s.n, 61 + n) - 0.5
+        xb, yb = s.vec(x), s.vec(y)
+        d1, d2 = s.dot(xb, yb, n), s.dot(xb, yb, n)
+        exact = math.fsum(x[:n] * y[:n])
+        bound = n * 2.0 ** -53 * float(np.abs(x[:n] * y[:n]).sum())
+        print(f"n={n}: dot {d1:.17e} fsum {exact:.17e} err {abs(d1 - exact):.2e} bound {bound:.2e}")
+        assert d1 == d2
+        assert abs(d1 - exact) <= bound
