@@ -40,6 +40,13 @@ struct KApply {
   const int* done;
 };
 
+// Loads of arrays each cell reads exactly once (no neighbour reuse): nontemporal, so they
+// do not displace the reused x/dut/face planes from the caches.
+template <typename T>
+__device__ __forceinline__ T ld_once(const T* p, long m) {
+  return __builtin_nontemporal_load(p + m);
+}
+
 // One tile of the z-march apply: a thread owns one column (i, j) of tile L and walks
 // its planes upwards, carrying the cells below/at/above and the z-face coefficients in
 // registers, so each level loads only the next cell up and the four horizontal
@@ -142,15 +149,16 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
         CT dm = 0, vm = 0, kzm = 0, tzm = 0, gzm = 0;
         if (k0 > 0) {
           const long m = c - sz;
-          xm = LX(m); dm = dut[m]; vm = vol[m]; kzm = kz[m]; tzm = tz[m]; gzm = gzz[m];
+          xm = LX(m); dm = dut[m]; vm = ld_once(vol, m); kzm = ld_once(kz, m);
+          tzm = ld_once(tz, m); gzm = ld_once(gzz, m);
         }
         V2<CT> x0 = LX(c);
-        CT d0 = dut[c], v0 = vol[c];
+        CT d0 = dut[c], v0 = ld_once(vol, c);
         for (int k = k0; k < k1; ++k, c += sz) {
           const long mU = k + 1 < nz ? c + sz : c;
           const V2<CT> xp = LX(mU);
-          const CT dp = dut[mU], vp = vol[mU];
-          const CT kz0 = kz[c], tz0 = tz[c], gz0 = gzz[c];
+          const CT dp = dut[mU], vp = ld_once(vol, mU);
+          const CT kz0 = ld_once(kz, c), tz0 = ld_once(tz, c), gz0 = ld_once(gzz, c);
           const long mW = c + dW, mE = c + dE, mS = c + dS, mN = c + dN;
           const V2<CT> xW = LX(mW), xE = LX(mE), xS = LX(mS), xN = LX(mN);
           V2<VU> u0{0, 0};
