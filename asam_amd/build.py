@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libasam_sound.so")
-SOURCES = ["setup.hip", "apply.hip", "line.hip", "blas.hip", "solver.hip", "batch.hip"]
+SOURCES = ["setup.hip", "apply.hip", "line.hip", "blas.hip", "solver.hip", "batch.hip",
+           "project.hip"]
 
 
 def hipcc() -> str:

@@ -189,4 +189,21 @@ void launch_gm_backsolve(hipStream_t s, KState* st);
 
 void launch_copy16(hipStream_t s, long n16, const void* a, void* b);
 
+// ------------------------------------------------------------------ either side of the solve
+// Geometry and weights the right-hand side and the post-solve update need (device).
+struct ProjInputs {
+  int grad_full;
+  double beta0_dt, grav;
+  const double *FU, *FV, *FW;     // face areas, shapes (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1)
+  const double *thu, *thv, *thw;  // face theta, same shapes
+  const double *V, *dut, *dx, *dy, *dz;
+};
+// b = bd * div(F theta_f u) - V incr_rt (and TB: bd * div(F u) - V incr_r)
+void launch_rhs(hipStream_t s, const Geom& g, const ProjInputs& in, double* b, const double* U,
+                const double* V, const double* W, const double* incr_rt, const double* incr_r);
+// face velocities -= grad(DUT x1) (+ TB buoyancy), increments -= bd div(...)/V, x1 *= DUT,
+// x2 *= DUR (= 0); incr_r may be null
+void launch_update(hipStream_t s, const Geom& g, const ProjInputs& in, double* x, double* U,
+                   double* V, double* W, double* incr_rt, double* incr_r);
+
 }  // namespace asam

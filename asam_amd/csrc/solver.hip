@@ -29,6 +29,9 @@ struct asam_sound_ctx {
   Coef<double> c64;
   Coef<float> c32;
   double* thfaces = nullptr;  // face theta, FU|FV|FW shapes concatenated
+  double *fu = nullptr, *fv = nullptr, *fw = nullptr;       // face areas (set-up input)
+  double *dxv = nullptr, *dyv = nullptr, *dzv = nullptr;    // cell spacings
+  double* pj_scratch = nullptr;  // host-array rhs/update staging (allocated on first use)
   double* partial = nullptr;
   KState* st = nullptr;
   KState* st_host = nullptr;  // pinned
@@ -307,8 +310,11 @@ static int setup_common(asam_sound_ctx* c, const double* FU, const double* FV, c
   launch_convert(s, g, c->c64, c->c32);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(s));
-  dfree(c, dFU); dfree(c, dFV); dfree(c, dFW); dfree(c, dV); dfree(c, ddx); dfree(c, ddy);
-  dfree(c, ddz);
+  dfree(c, dV);
+  // the face areas and spacings stay for the right-hand side and the post-solve update
+  dfree(c, c->fu); dfree(c, c->fv); dfree(c, c->fw);
+  dfree(c, c->dxv); dfree(c, c->dyv); dfree(c, c->dzv);
+  c->fu = dFU; c->fv = dFV; c->fw = dFW; c->dxv = ddx; c->dyv = ddy; c->dzv = ddz;
   c->ready = 1;
   return 0;
 }
@@ -833,6 +839,123 @@ extern "C" int asam_sound_theta_vjp_device(asam_sound_ctx* c, double* grad, cons
 }
 
 // ------------------------------------------------------------------ device helpers
+// ------------------------------------------------------------------ either side of the solve
+static ProjInputs proj_inputs(const asam_sound_ctx* c) {
+  ProjInputs in;
+  const Geom& g = c->g;
+  const long nfu = (long)(g.nx + 1) * g.ny * g.nz, nfv = (long)g.nx * (g.ny + 1) * g.nz;
+  in.grad_full = c->grid.grad_full;
+  in.beta0_dt = c->grid.beta0_dt;
+  in.grav = c->grid.grav;
+  in.FU = c->fu; in.FV = c->fv; in.FW = c->fw;
+  in.thu = c->thfaces; in.thv = c->thfaces + nfu; in.thw = c->thfaces + nfu + nfv;
+  in.V = c->c64.vol; in.dut = c->c64.dut;
+  in.dx = c->dxv; in.dy = c->dyv; in.dz = c->dzv;
+  return in;
+}
+
+extern "C" int asam_sound_rhs_device(asam_sound_ctx* c, double* b, const double* U,
+                                     const double* V, const double* W, const double* incr_rt,
+                                     const double* incr_r) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "rhs before setup");
+  if (!b || !U || !V || !W || !incr_rt || (c->g.np1 == 2 && !incr_r))
+    return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  launch_rhs(c->stream, c->g, proj_inputs(c), b, U, V, W, incr_rt, incr_r);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int asam_sound_update_device(asam_sound_ctx* c, double* x, double* U, double* V,
+                                        double* W, double* incr_rt, double* incr_r) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "update before setup");
+  if (!x || !U || !V || !W || !incr_rt) return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  launch_update(c->stream, c->g, proj_inputs(c), x, U, V, W, incr_rt, incr_r);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// Host-array forms: staged through one device block [x|b, U, V, W, incr_rt, incr_r].
+static int pj_stage(asam_sound_ctx* c, double** xb, double** U, double** V, double** W,
+                    double** it, double** ir) {
+  const Geom& g = c->g;
+  const long nfu = (long)(g.nx + 1) * g.ny * g.nz, nfv = (long)g.nx * (g.ny + 1) * g.nz,
+             nfw = (long)g.nx * g.ny * (g.nz + 1);
+  if (!c->pj_scratch) {
+    int rc = dalloc(c, &c->pj_scratch, c->n + nfu + nfv + nfw + 2 * g.N);
+    if (rc) return rc;
+  }
+  *xb = c->pj_scratch;
+  *U = *xb + c->n;
+  *V = *U + nfu;
+  *W = *V + nfv;
+  *it = *W + nfw;
+  *ir = *it + g.N;
+  return 0;
+}
+
+extern "C" int asam_sound_rhs(asam_sound_ctx* c, double* b, const double* U, const double* V,
+                              const double* W, const double* incr_rt, const double* incr_r) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "rhs before setup");
+  if (!b || !U || !V || !W || !incr_rt || (c->g.np1 == 2 && !incr_r))
+    return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *db, *dU, *dV, *dW, *dit, *dir;
+  int rc = pj_stage(c, &db, &dU, &dV, &dW, &dit, &dir);
+  if (rc) return rc;
+  const Geom& g = c->g;
+  const long nfu = (long)(g.nx + 1) * g.ny * g.nz, nfv = (long)g.nx * (g.ny + 1) * g.nz,
+             nfw = (long)g.nx * g.ny * (g.nz + 1);
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(dU, U, nfu * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dV, V, nfv * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dW, W, nfw * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dit, incr_rt, g.N * 8, hipMemcpyHostToDevice, s));
+  if (incr_r) HIPCHK(c, hipMemcpyAsync(dir, incr_r, g.N * 8, hipMemcpyHostToDevice, s));
+  rc = asam_sound_rhs_device(c, db, dU, dV, dW, dit, incr_r ? dir : nullptr);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(b, db, c->n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
+}
+
+extern "C" int asam_sound_update(asam_sound_ctx* c, double* x, double* U, double* V, double* W,
+                                 double* incr_rt, double* incr_r) {
+  if (!c) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "update before setup");
+  if (!x || !U || !V || !W || !incr_rt) return fail(c, ASAM_E_INVALID, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *dx, *dU, *dV, *dW, *dit, *dir;
+  int rc = pj_stage(c, &dx, &dU, &dV, &dW, &dit, &dir);
+  if (rc) return rc;
+  const Geom& g = c->g;
+  const long nfu = (long)(g.nx + 1) * g.ny * g.nz, nfv = (long)g.nx * (g.ny + 1) * g.nz,
+             nfw = (long)g.nx * g.ny * (g.nz + 1);
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemcpyAsync(dx, x, c->n * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dU, U, nfu * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dV, V, nfv * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dW, W, nfw * 8, hipMemcpyHostToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(dit, incr_rt, g.N * 8, hipMemcpyHostToDevice, s));
+  if (incr_r) HIPCHK(c, hipMemcpyAsync(dir, incr_r, g.N * 8, hipMemcpyHostToDevice, s));
+  rc = asam_sound_update_device(c, dx, dU, dV, dW, dit, incr_r ? dir : nullptr);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(x, dx, c->n * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(U, dU, nfu * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(V, dV, nfv * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(W, dW, nfw * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipMemcpyAsync(incr_rt, dit, g.N * 8, hipMemcpyDeviceToHost, s));
+  if (incr_r) HIPCHK(c, hipMemcpyAsync(incr_r, dir, g.N * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  return 0;
+}
+
 extern "C" int asam_dev_alloc(asam_sound_ctx* c, void** ptr, size_t bytes) {
   if (!c || !ptr) return ASAM_E_INVALID;
   HIPCHK(c, hipSetDevice(c->device));
@@ -892,7 +1015,7 @@ extern "C" int asam_dev_mass(asam_sound_ctx* c, double* y, const double* x) {
 // ------------------------------------------------------------------ measurement
 extern "C" int asam_sound_time_kernel(asam_sound_ctx* c, int which, int fp32, int reps,
                                       double* ms) {
-  if (!c || !ms || reps < 1) return ASAM_E_INVALID;
+  if (!c || !ms || reps < 1 || which < 0 || which > 4) return ASAM_E_INVALID;
   if (!c->ready) return fail(c, ASAM_E_STATE, "before setup");
   HIPCHK(c, hipSetDevice(c->device));
   Enq e(c, 0);
@@ -903,9 +1026,18 @@ extern "C" int asam_sound_time_kernel(asam_sound_ctx* c, int which, int fp32, in
   launch_fill(c->stream, c->n, prec, y, 0.5);
   launch_fill(c->stream, c->n, prec, fp32 ? (void*)c->r32 : (void*)c->r, 0.25);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *px = nullptr, *pU = nullptr, *pV = nullptr, *pW = nullptr, *pit = nullptr,
+         *pir = nullptr;
+  if (which == 3 || which == 4) {  // FP64 only; staging block zero-filled (values stay 0)
+    int rc = pj_stage(c, &px, &pU, &pV, &pW, &pit, &pir);
+    if (rc) return rc;
+    launch_fill(c->stream, (long)(pir + c->g.N - px), 0, px, 0.0);
+  }
   auto once = [&]() {
     if (which == 0) e.apply(prec, prec, prec, prec, x, y, nullptr, nullptr, nullptr, 0, nullptr);
     else if (which == 1) e.line(prec, LINE_PLAIN, x, nullptr, nullptr, y, nullptr, nullptr);
+    else if (which == 3) launch_rhs(c->stream, c->g, proj_inputs(c), px, pU, pV, pW, pit, pir);
+    else if (which == 4) launch_update(c->stream, c->g, proj_inputs(c), px, pU, pV, pW, pit, pir);
     else {
       memset(c->st_host, 0, sizeof(KState));
       launch_bicg_update(c->stream, c->n, prec, x, y, y, fp32 ? (void*)c->r32 : (void*)c->r, y,

@@ -126,6 +126,23 @@ module asam_sound_iface
       real(c_double), intent(in) :: x(*)
     end function
 
+    integer(c_int) function asam_sound_rhs(ctx, b, U, V, W, incr_rt, incr_r) &
+        bind(c, name="asam_sound_rhs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: b(*)
+      real(c_double), intent(in) :: U(*), V(*), W(*), incr_rt(*)
+      type(c_ptr), value :: incr_r
+    end function
+
+    integer(c_int) function asam_sound_update(ctx, x, U, V, W, incr_rt, incr_r) &
+        bind(c, name="asam_sound_update")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: x(*), U(*), V(*), W(*), incr_rt(*)
+      type(c_ptr), value :: incr_r
+    end function
+
     integer(c_int) function asam_sound_psolve(ctx, y, x) bind(c, name="asam_sound_psolve")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -12,7 +12,8 @@ module AsamSound_Mod
   implicit none
   private
   public :: PressureVelocity, AsamSoundInit, AsamJacAccGrav, SolveSound, SolveSoundT, &
-            MatVec1, PSolve1, AsamSoundFinalize, AsamLastInfo, AsamStatus
+            MatVec1, PSolve1, Rhs, UpdateVelocity, AsamSoundFinalize, AsamLastInfo, &
+            AsamStatus
 
   !> The p component of the reference's TYPE(PressureVelocity) (VecPrVel_Mod.f90:11-18);
   !> one closed block, so the interface velocities are absent.
@@ -108,6 +109,36 @@ contains
     rc = asam_sound_psolve(ctx, b(1)%p, x(1)%p)
     call check(rc, 'asam_sound_psolve')
   end subroutine PSolve1
+
+  !> b = Rhs of the stage system (Rhs_Mod.f90:110-157): U, V, W are the predicted face
+  !> velocities Vel%uF(0:nx,ny,nz), %vF(nx,0:ny,nz), %wF(nx,ny,0:nz); IncrRhoTheta and
+  !> IncrRho the stage increments of rho*theta and rho (interior cells).
+  subroutine Rhs(b, U, V, W, IncrRhoTheta, IncrRho)
+    type(PressureVelocity), dimension(:) :: b
+    real(c_double), intent(in) :: U(:,:,:), V(:,:,:), W(:,:,:), IncrRhoTheta(:,:,:)
+    real(c_double), intent(in), optional, target, contiguous :: IncrRho(:,:,:)
+    type(c_ptr) :: ir
+    integer(c_int) :: rc
+    ir = c_null_ptr
+    if (present(IncrRho)) ir = c_loc(IncrRho)
+    rc = asam_sound_rhs(ctx, b(1)%p, U, V, W, IncrRhoTheta, ir)
+    call check(rc, 'asam_sound_rhs')
+  end subroutine Rhs
+
+  !> Post-solve update (UpdateVelocity, Rhs_Mod.f90:589-952): x(1)%p(1,...) becomes the
+  !> pressure increment DUT*x1, the face velocities get the pressure-gradient (and, with
+  !> two unknowns, buoyancy) correction, and the increments the corrected flux divergence.
+  subroutine UpdateVelocity(x, U, V, W, IncrRhoTheta, IncrRho)
+    type(PressureVelocity), dimension(:) :: x
+    real(c_double), intent(inout) :: U(:,:,:), V(:,:,:), W(:,:,:), IncrRhoTheta(:,:,:)
+    real(c_double), intent(inout), optional, target, contiguous :: IncrRho(:,:,:)
+    type(c_ptr) :: ir
+    integer(c_int) :: rc
+    ir = c_null_ptr
+    if (present(IncrRho)) ir = c_loc(IncrRho)
+    rc = asam_sound_update(ctx, x(1)%p, U, V, W, IncrRhoTheta, ir)
+    call check(rc, 'asam_sound_update')
+  end subroutine UpdateVelocity
 
   subroutine AsamSoundFinalize()
     integer(c_int) :: rc

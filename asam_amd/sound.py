@@ -118,6 +118,10 @@ def load(path: str = LIB_PATH):
         "asam_sound_time_copy": (i, [_vp, C.c_size_t, i, dp]),
         "asam_sound_set_tuning": (i, [_vp, C.POINTER(Tuning)]),
         "asam_sound_profile_read": (i, [_vp, dp, dp, ip]),
+        "asam_sound_rhs": (i, [_vp] + [dp] * 6),
+        "asam_sound_rhs_device": (i, [_vp] * 7),
+        "asam_sound_update": (i, [_vp] + [dp] * 6),
+        "asam_sound_update_device": (i, [_vp] * 7),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -306,6 +310,35 @@ class SoundSolver:
         else:
             buf.upload(_c(a))
         return buf
+
+    # ------------------------------------------------------ either side of the solve
+    def Rhs(self, U, V, W, incr_rt, incr_r=None):
+        """b of the stage system from predicted face velocities U (nz,ny,nx+1), V
+        (nz,ny+1,nx), W (nz+1,ny,nx) and the stage increments of rho*theta and rho
+        (reference Rhs, MATRIX/Rhs_Mod.f90:110-157)."""
+        b = np.empty(self.n)
+        a = [_c(v) for v in (U, V, W, incr_rt)]
+        ir = _c(incr_r) if incr_r is not None else None
+        self._chk(self.L.asam_sound_rhs(self.h, _p(b), *[_p(v) for v in a],
+                                        _p(ir) if ir is not None else None))
+        return b
+
+    def UpdateVelocity(self, x, U, V, W, incr_rt, incr_r=None):
+        """Post-solve update (reference UpdateVelocity, MATRIX/Rhs_Mod.f90:589-952);
+        returns updated copies (x, U, V, W, incr_rt, incr_r)."""
+        x, U, V, W, it = (np.array(v, dtype=np.float64) for v in (x, U, V, W, incr_rt))
+        ir = None if incr_r is None else np.array(incr_r, dtype=np.float64)
+        self._chk(self.L.asam_sound_update(self.h, _p(x), _p(U), _p(V), _p(W), _p(it),
+                                           _p(ir) if ir is not None else None))
+        return x, U, V, W, it, ir
+
+    def rhs_device(self, b, U, V, W, it, ir=None):
+        self._chk(self.L.asam_sound_rhs_device(self.h, b.ptr, U.ptr, V.ptr, W.ptr, it.ptr,
+                                               ir.ptr if ir is not None else None))
+
+    def update_device(self, x, U, V, W, it, ir=None):
+        self._chk(self.L.asam_sound_update_device(self.h, x.ptr, U.ptr, V.ptr, W.ptr, it.ptr,
+                                                  ir.ptr if ir is not None else None))
 
     def zero(self, buf: DeviceBuffer):
         self._chk(self.L.asam_dev_memset(self.h, buf.ptr, 0, buf.nbytes))

@@ -293,3 +293,26 @@ def ensemble_members(n_members: int = 64, seed: int = 5, np1: int = 1, size_scal
                                 bubble_amp=amp, np1=np1, seed=seed + 1 + m,
                                 name=f"member{m}"))
     return out
+
+
+def stage_inputs(prob: SoundProblem, seed: int = 11, wind: float = 10.0, noise: float = 1.0,
+                 incr_scale: float = 1e-3):
+    """Predicted face velocities and stage increments for one stage projection (the inputs
+    Rhs takes besides the operator): u = wind + noise N(0,1), v, w = noise N(0,1) on open
+    faces; faces on the walls and faces closed by the terrain (F = 0) carry no flow.
+    Increments of rho*theta and rho: incr_scale N(0,1) on live cells.
+    Returns U (nz,ny,nx+1), V (nz,ny+1,nx), W (nz+1,ny,nx), incr_rt, incr_r (nz,ny,nx)."""
+    g = prob.grid
+    U = wind + noise * normal(prob.FU.size, seed).reshape(prob.FU.shape)
+    V = noise * normal(prob.FV.size, seed + 1).reshape(prob.FV.shape)
+    W = noise * normal(prob.FW.size, seed + 2).reshape(prob.FW.shape)
+    U[:, :, [0, -1]] = 0.0
+    V[:, [0, -1], :] = 0.0
+    W[[0, -1]] = 0.0
+    U[prob.FU <= 0.0] = 0.0
+    V[prob.FV <= 0.0] = 0.0
+    W[prob.FW <= 0.0] = 0.0
+    live = prob.VolC > 0.0
+    incr_rt = np.where(live, incr_scale * normal(g.ncell, seed + 3).reshape(live.shape), 0.0)
+    incr_r = np.where(live, incr_scale * normal(g.ncell, seed + 4).reshape(live.shape), 0.0)
+    return U, V, W, incr_rt, incr_r

@@ -283,6 +283,49 @@ def ensemble_bench(a):
     print(json.dumps(line))
 
 
+def proj_bytes(p):
+    """Algorithmic bytes of the right-hand side and of the post-solve update (both kernels)
+    at problem p: every distinct array element read or written once."""
+    g = p.grid
+    N, np1 = g.ncell, p.np1
+    nf = (g.nx + 1) * g.ny * g.nz + g.nx * (g.ny + 1) * g.nz + g.nx * g.ny * (g.nz + 1)
+    rhs = 8 * (3 * nf + N * (2 + (np1 == 2)) + N * np1)
+    faces = 8 * (N * (np1 + 2) + nf + 2 * nf)           # x, DUT, V; F; u read + write
+    cells = 8 * (3 * nf + N * (1 + 2 + np1 + 1) + N * (2 + np1))  # u, F, theta_f; V, incr, x, DUT
+    return rhs, faces + cells
+
+
+def projection_leg(s, p, steps, tol):
+    """One stage projection per step on resident inputs: Rhs -> SolveSound -> UpdateVelocity
+    (the three calls ProjectVelFace makes, reference ADVECT/IntSub_Mod.f90:40-70)."""
+    from asam_amd import workload
+    U, V, W, it, ir = workload.stage_inputs(p, seed=41)
+    src = [s.alloc(a.size * 8).upload(np.ascontiguousarray(a)) for a in (U, V, W, it, ir)]
+    work = [s.alloc(a.size * 8) for a in (U, V, W, it, ir)]
+    bbuf, xbuf = s.vec(), s.vec()
+    times = []
+    for step in range(steps + 1):
+        for d, q in zip(work, src):
+            s._chk(s.L.asam_dev_memset(s.h, d.ptr, 0, d.nbytes))
+            s.axpby(1.0, q, 0.0, d, n=d.nbytes // 8)
+        s.zero(xbuf)
+        t0 = time.perf_counter()
+        s.rhs_device(bbuf, *work)
+        s.solve_device(xbuf, bbuf, 500, tol)
+        s.update_device(xbuf, *work)
+        if step:
+            times.append(time.perf_counter() - t0)
+    rb, ub = proj_bytes(p)
+    ms_r = s.time_kernel(3, False, 20)
+    ms_u = s.time_kernel(4, False, 20)
+    return {"stage_ms": round(1e3 * float(np.median(times)), 4),
+            "stages_per_s": round(1.0 / float(np.median(times)), 2),
+            "rhs": {"avg_launch_us": round(ms_r * 1e3, 1), "bytes": rb,
+                    "GBs": round(rb / (ms_r * 1e-3) / 1e9, 1)},
+            "update": {"avg_us_two_kernels": round(ms_u * 1e3, 1), "bytes": ub,
+                       "GBs": round(ub / (ms_u * 1e-3) / 1e9, 1)}}
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -323,6 +366,8 @@ def main():
     true_rel = float(np.linalg.norm(r) / np.linalg.norm(p.b))
     diff_vs_fp64_gpu = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
+    line_proj = projection_leg(s, p, max(2, a.steps // 4), a.tol)
+
     roof, dom = roofline_from_profile(profmx, a.np1)
     if roof is not None:
         roof["traffic"] = read_traffic(dom, traffic_workload(a.np1, a.tol))
@@ -352,6 +397,7 @@ def main():
                   "rel_diff_vs_fp64_gpu": diff_vs_fp64_gpu,
                   "instrumented_ms_per_step": round(tmx_i * 1e3, 4)},
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+        "projection": line_proj,
         "kernel_timing": "HIP events around every launch of the instrumented replay of the timed solves",
         "kernels_mixed": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
                               "launches_per_solve": v["launches"] / a.steps,

@@ -166,6 +166,29 @@ int asam_sound_solve_T_device(asam_sound_ctx *ctx, double *lambda, const double 
 int asam_sound_theta_vjp_device(asam_sound_ctx *ctx, double *grad, const double *mu,
                                 const double *x);
 
+/* ---- either side of the solve (one closed block, dry 'Density' weights) -- */
+/* Right-hand side of the stage system (reference Rhs, MATRIX/Rhs_Mod.f90:110-157):
+ *   b1 = beta0_dt * DivergenceS(F, theta_f, u) - V incr_rt     (FacAnela >= 1; else no V term)
+ *   b2 = beta0_dt * Divergence(F, u) - V incr_r                 (np1 = 2)
+ * U (nx+1,ny,nz), V (nx,ny+1,nz), W (nx,ny,nz+1): predicted face velocities (reference
+ * Vel%uF/vF/wF); incr_rt, incr_r (nx,ny,nz): stage increments of rho*theta and rho
+ * (incr_r may be NULL when np1 = 1); b: np1*nx*ny*nz.  Uses the face areas, face theta and
+ * volumes of the last set-up. */
+int asam_sound_rhs(asam_sound_ctx *ctx, double *b, const double *U, const double *V,
+                   const double *W, const double *incr_rt, const double *incr_r);
+int asam_sound_rhs_device(asam_sound_ctx *ctx, double *b, const double *U, const double *V,
+                          const double *W, const double *incr_rt, const double *incr_r);
+/* Post-solve update (reference UpdateVelocity, MATRIX/Rhs_Mod.f90:589-952): with x the
+ * solution, x1 *= DUT (pressure); np1 = 2: interior w faces -= beta0_dt Grav
+ * (V_k x2_k + V_k+1 x2_k+1)/(V_k + V_k+1 + Eps), then x2 *= DUR (= 0 for dry weights);
+ * interior faces -= the pressure-gradient correction (GradFull or Dual form); then
+ * incr_rt -= beta0_dt DivergenceS(u)/V and incr_r -= beta0_dt Divergence(u)/V with the
+ * corrected velocities (incr_r may be NULL).  All arrays in/out; wall faces unchanged. */
+int asam_sound_update(asam_sound_ctx *ctx, double *x, double *U, double *V, double *W,
+                      double *incr_rt, double *incr_r);
+int asam_sound_update_device(asam_sound_ctx *ctx, double *x, double *U, double *V, double *W,
+                             double *incr_rt, double *incr_r);
+
 /* ---- device memory helpers (on the context's device) ------------------- */
 int asam_dev_alloc(asam_sound_ctx *ctx, void **ptr, size_t bytes);
 int asam_dev_free(asam_sound_ctx *ctx, void *ptr);
@@ -183,7 +206,8 @@ int asam_dev_mass(asam_sound_ctx *ctx, double *y, const double *x);
 /* ---- measurement hooks -------------------------------------------------- */
 /* Average device time (ms, HIP events on the context's stream) of one launch of the
  * stencil apply (which=0), line preconditioner (which=1) or fused BiCGStab update
- * (which=2), in FP64 (fp32=0) or FP32 (fp32=1), over reps launches. */
+ * (which=2), in FP64 (fp32=0) or FP32 (fp32=1), or of the FP64 right-hand side (which=3)
+ * or post-solve update (which=4, both of its kernels), over reps launches. */
 int asam_sound_time_kernel(asam_sound_ctx *ctx, int which, int fp32, int reps, double *ms);
 /* Average time (ms) of a 16-byte-vector streaming copy of `bytes` bytes (reads bytes, writes
  * bytes): the achievable-bandwidth and profiler-counter calibration kernel. */

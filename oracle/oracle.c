@@ -722,3 +722,140 @@ int orc_gmres(const orc_op *A, double *x, const double *b, int maxiter, double t
   free(v); free(w);
   return total;
 }
+
+
+/* ------------------------------------------------------------------ caller side */
+/* Face-flux divergence of one cell (DivergenceS / Divergence, Rhs_Mod.f90:13-82): the
+ * reference's term order, west, east, south, north, bottom, top; th* NULL: no weight. */
+static double div_cell(const orc_grid *g, int i, int j, int k, const double *FU,
+                       const double *FV, const double *FW, const double *thu,
+                       const double *thv, const double *thw, const double *U,
+                       const double *Vv, const double *W) {
+  const long nx = g->nx, ny = g->ny;
+  const long uw = i + (nx + 1) * (j + ny * k), ue = uw + 1;
+  const long vs = i + nx * (j + (ny + 1) * k), vn = vs + nx;
+  const long wb = i + nx * (j + ny * k), wt = wb + nx * ny;
+  double s;
+  if (thu) {
+    s = -FU[uw] * thu[uw] * U[uw];
+    s += FU[ue] * thu[ue] * U[ue];
+    s += -FV[vs] * thv[vs] * Vv[vs];
+    s += FV[vn] * thv[vn] * Vv[vn];
+    s += -FW[wb] * thw[wb] * W[wb];
+    s += FW[wt] * thw[wt] * W[wt];
+  } else {
+    s = -FU[uw] * U[uw];
+    s += FU[ue] * U[ue];
+    s += -FV[vs] * Vv[vs];
+    s += FV[vn] * Vv[vn];
+    s += -FW[wb] * W[wb];
+    s += FW[wt] * W[wt];
+  }
+  return s;
+}
+
+void orc_rhs(const orc_grid *g, const double *FU, const double *FV, const double *FW,
+             const double *V, const double *DTU, const double *DTV, const double *DTW,
+             const double *U, const double *Vv, const double *W, const double *incr_rt,
+             const double *incr_r, double *b) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, np1 = g->np1;
+  const double bd = g->beta0_dt;
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long c = i + (long)nx * (j + (long)ny * k);
+        double b1 = bd * div_cell(g, i, j, k, FU, FV, FW, DTU, DTV, DTW, U, Vv, W);
+        if (g->fac_anela >= 1.0) b1 -= V[c] * incr_rt[c];
+        b[np1 * c] = b1;
+        if (np1 == 2)
+          b[2 * c + 1] = bd * div_cell(g, i, j, k, FU, FV, FW, NULL, NULL, NULL, U, Vv, W) -
+                         V[c] * incr_r[c];
+      }
+}
+
+void orc_update(const orc_grid *g, const double *FU, const double *FV, const double *FW,
+                const double *V, const double *dx, const double *dy, const double *dz,
+                const double *DTU, const double *DTV, const double *DTW, const double *DUT,
+                double *x, double *U, double *Vv, double *W, double *incr_rt, double *incr_r) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, np1 = g->np1;
+  const long N = (long)nx * ny * nz, sy = nx, sz = (long)nx * ny;
+  const double bd = g->beta0_dt;
+  /* x1 = x1*DUT (:611) */
+  for (long c = 0; c < N; ++c) x[np1 * c] = x[np1 * c] * DUT[c];
+  if (np1 == 2) {
+    /* buoyancy correction of the interior w faces (:613-621), then x2 = x2*DUR = 0 */
+    for (int k = 0; k + 1 < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long c = i + sy * j + sz * k, u = c + sz;
+          const double t = bd * g->grav * (V[c] * x[2 * c + 1] + V[u] * x[2 * u + 1]) /
+                           (V[u] + V[c] + EPS);
+          W[c + sz] = W[c + sz] - t;
+        }
+    for (long c = 0; c < N; ++c) x[2 * c + 1] = x[2 * c + 1] * 0.0;
+  }
+  /* interior faces: u = u - grad p (GradFull :679-745 / Dual :771-838, Cartesian metric) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long c = i + sy * j + sz * k;
+        const double p1 = x[np1 * c], p2 = np1 == 2 ? x[2 * c + 1] : 0.0;
+        if (i + 1 < nx) {
+          const long f = (i + 1) + (long)(nx + 1) * (j + (long)ny * k), n = c + 1;
+          const double duu = FU[f] / (FU[f] + EPS);
+          double dp = x[np1 * n] - p1;
+          if (np1 == 2) dp = dp + (x[2 * n + 1] - p2);
+          const double t = g->grad_full ? 2.0 * bd * duu * dp / (dx[i + 1] + dx[i])
+                                        : 2.0 * bd * FU[f] * duu * dp / (V[n] + V[c] + EPS);
+          U[f] = U[f] - t;
+        }
+        if (j + 1 < ny) {
+          const long f = i + (long)nx * ((j + 1) + (long)(ny + 1) * k), n = c + sy;
+          const double duv = FV[f] / (FV[f] + EPS);
+          double dp = x[np1 * n] - p1;
+          if (np1 == 2) dp = dp + (x[2 * n + 1] - p2);
+          const double t = g->grad_full ? 2.0 * bd * duv * dp / (dy[j + 1] + dy[j])
+                                        : 2.0 * bd * FV[f] * duv * dp / (V[n] + V[c] + EPS);
+          Vv[f] = Vv[f] - t;
+        }
+        if (k + 1 < nz) {
+          const long f = c + sz, n = c + sz;
+          const double duw = FW[f] / (FW[f] + EPS);
+          double dp = x[np1 * n] - p1;
+          if (np1 == 2) dp = dp + (x[2 * n + 1] - p2);
+          if (g->grad_full) {
+            const double t = 2.0 * bd * duw * dp / (dz[k + 1] + dz[k]);
+            /* MultiTriT scales the w correction by FW/(FW+Eps) (:705-713) */
+            W[f] = np1 == 1 ? W[f] - FW[f] / (FW[f] + EPS) * t : W[f] - t;
+          } else {
+            W[f] = W[f] - 2.0 * bd * FW[f] * duw * dp / (V[n] + V[c] + EPS);
+          }
+        }
+      }
+  /* increments from the corrected face fluxes (:840-873) */
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long c = i + sy * j + sz * k;
+        const long uw = i + (long)(nx + 1) * (j + (long)ny * k), ue = uw + 1;
+        const long vs = i + (long)nx * (j + (long)(ny + 1) * k), vn = vs + nx;
+        const long wb = c, wt = c + sz;
+        double s = FU[ue] * DTU[ue] * U[ue];
+        s += -FU[uw] * DTU[uw] * U[uw];
+        s += FV[vn] * DTV[vn] * Vv[vn];
+        s += -FV[vs] * DTV[vs] * Vv[vs];
+        s += FW[wt] * DTW[wt] * W[wt];
+        s += -FW[wb] * DTW[wb] * W[wb];
+        incr_rt[c] = incr_rt[c] - bd * s / (V[c] + EPS);
+        if (incr_r) {
+          double r = FU[ue] * U[ue];
+          r += -FU[uw] * U[uw];
+          r += FV[vn] * Vv[vn];
+          r += -FV[vs] * Vv[vs];
+          r += FW[wt] * W[wt];
+          r += -FW[wb] * W[wb];
+          incr_r[c] = incr_r[c] - bd * r / (V[c] + EPS);
+        }
+      }
+}

@@ -69,6 +69,25 @@ int orc_gmres(const orc_op *A, double *x, const double *b, int maxiter, double t
               int transpose, double *rel_res);
 
 double orc_dot(long n, const double *x, const double *y);
+/* Right-hand side of the stage system (Rhs, MATRIX/Rhs_Mod.f90:110-157, one closed block,
+ * dry 'Density' weights so DTT = 1): b1 = beta0dt * DivergenceS(F, theta_f, u)
+ * (:46-82) - V * incr_rt (when fac_anela >= 1); TB: b2 = beta0dt * Divergence(F, u)
+ * (:13-43) - V * incr_r.  U (nx+1,ny,nz), Vv (nx,ny+1,nz), W (nx,ny,nz+1) face
+ * velocities; incr_rt, incr_r cell increments (incr_r unused for np1 = 1). */
+void orc_rhs(const orc_grid *g, const double *FU, const double *FV, const double *FW,
+             const double *V, const double *DTU, const double *DTV, const double *DTW,
+             const double *U, const double *Vv, const double *W, const double *incr_rt,
+             const double *incr_r, double *b);
+/* Post-solve update (UpdateVelocity, MATRIX/Rhs_Mod.f90:589-952, one closed block, dry
+ * 'Density': DUR = 0, DTT = 1, Cartesian metric): x1 *= DUT; TB: buoyancy correction of
+ * the interior w faces with x2, then x2 *= DUR (= 0); pressure-gradient correction of the
+ * interior faces (GradFull or Dual form); then incr_rt -= beta0dt DivergenceS(u)/V and
+ * incr_r -= beta0dt Divergence(u)/V with the corrected velocities (incr_r may be NULL). */
+void orc_update(const orc_grid *g, const double *FU, const double *FV, const double *FW,
+                const double *V, const double *dx, const double *dy, const double *dz,
+                const double *DTU, const double *DTV, const double *DTW, const double *DUT,
+                double *x, double *U, double *Vv, double *W, double *incr_rt, double *incr_r);
+
 void orc_set_threads(int n);
 int orc_get_threads(void);
 

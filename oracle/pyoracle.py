@@ -57,6 +57,8 @@ def lib():
             getattr(L, f).restype = C.c_int
             getattr(L, f).argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                       C.c_int, _dp]
+        L.orc_rhs.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 13
+        L.orc_update.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 16
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -83,6 +85,7 @@ class Oracle:
                          prob.grav, 1.0)
         self._keep = [_c(prob.FU), _c(prob.FV), _c(prob.FW), _c(prob.VolC), _c(prob.dxv),
                       _c(prob.dyv), _c(prob.dzv)]
+        self._w = None if weights is None else [_c(a) for a in weights]
         if weights is None:
             st = [_c(prob.rho), _c(prob.rhotheta), _c(prob.T)]
             self._h = L.orc_assemble(C.byref(self.g), *[_p(a) for a in self._keep + st])
@@ -99,6 +102,35 @@ class Oracle:
                 self._h = None
         except Exception:
             pass
+
+    def _face_weights(self):
+        if self._w is None:
+            self._w = [_c(a) for a in self.weights()]
+        return self._w
+
+    def rhs(self, U, V, W, incr_rt, incr_r=None):
+        """b of the stage system from predicted face velocities and the stage increments
+        (Rhs, Rhs_Mod.f90:110-157)."""
+        p = self.prob
+        DTU, DTV, DTW, _ = self._face_weights()
+        b = np.empty(self.n)
+        ir = _c(incr_r if incr_r is not None else np.zeros(self.N))
+        lib().orc_rhs(C.byref(self.g), _p(self._keep[0]), _p(self._keep[1]), _p(self._keep[2]),
+                      _p(self._keep[3]), _p(DTU), _p(DTV), _p(DTW), _p(_c(U)), _p(_c(V)),
+                      _p(_c(W)), _p(_c(incr_rt)), _p(ir), _p(b))
+        return b
+
+    def update(self, x, U, V, W, incr_rt, incr_r=None):
+        """UpdateVelocity (Rhs_Mod.f90:589-952): returns updated copies
+        (x, U, V, W, incr_rt, incr_r)."""
+        DTU, DTV, DTW, DUT = self._face_weights()
+        x, U, V, W, it = (np.array(a, dtype=np.float64) for a in (x, U, V, W, incr_rt))
+        ir = None if incr_r is None else np.array(incr_r, dtype=np.float64)
+        k = self._keep
+        lib().orc_update(C.byref(self.g), _p(k[0]), _p(k[1]), _p(k[2]), _p(k[3]), _p(k[4]),
+                         _p(k[5]), _p(k[6]), _p(DTU), _p(DTV), _p(DTW), _p(_c(DUT)), _p(x),
+                         _p(U), _p(V), _p(W), _p(it), _p(ir) if ir is not None else None)
+        return x, U, V, W, it, ir
 
     def weights(self):
         p, g = self.prob, self.prob.grid

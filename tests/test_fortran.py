@@ -102,3 +102,37 @@ def test_fortran_solve_sound_matches_oracle(np1, method, oracle_mod, tmp_path):
     err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
     print(f"Fortran SolveSound np1={np1} method={method}: err vs oracle {err:.3e}")
     assert err < 1e-10 and rel <= 1.0001e-8
+
+
+@needs_flang
+@pytest.mark.gpu
+@pytest.mark.parametrize("np1", [1, 2])
+def test_fortran_project_vel_face_matches_oracle(np1, oracle_mod, tmp_path):
+    """Rhs -> SolveSound -> UpdateVelocity from Fortran, against the oracle's sequence."""
+    ensure_built()
+    drv = os.path.join(FDIR, "_build", "drv_project_vel_face")
+    p = W.config(4, np1=np1, scale=0.5)
+    U, V, Wv, it, ir = W.stage_inputs(p, seed=51)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, p, method=0, maxiter=300, tol=1e-12)
+    with open(fin, "ab") as f:
+        for a in (U, V, Wv, it, ir):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+    r = subprocess.run([drv, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    orc = oracle_mod.Oracle(p)
+    b = orc.rhs(U, V, Wv, it, ir)
+    xo, mo, to, so = orc.bicgstab(b, 300, 1e-12)
+    ref = orc.update(xo, U, V, Wv, it, ir)
+    with open(fout, "rb") as f:
+        maxiter, = struct.unpack("<i", f.read(4))
+        tol, = struct.unpack("<d", f.read(8))
+        got = []
+        for a in ref:
+            got.append(np.frombuffer(f.read(a.nbytes), dtype="<f8").reshape(a.shape))
+    assert abs(abs(maxiter) - abs(mo)) <= 1
+    for nm, g_, o_ in zip(("x", "U", "V", "W", "incr_rt", "incr_r"), got, ref):
+        d = np.abs(g_ - o_).max() / max(np.abs(o_).max(), 1e-300)
+        print(f"np1={np1} {nm}: rel diff {d:.2e}")
+        assert d < 1e-10, nm
