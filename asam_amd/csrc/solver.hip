@@ -1061,6 +1061,91 @@ extern "C" int asam_sound_time_kernel(asam_sound_ctx* c, int which, int fp32, in
   return 0;
 }
 
+// Launch-shape autotuner.  Objective: device time of a fixed number of BiCGStab
+// iterations (every kernel and finalize of the iteration, in the order a solve runs
+// them, on this context's operator) in the precision the control selects.  Timing whole
+// iterations rather than isolated launches keeps each kernel's cache context the same as
+// in a solve.  Coordinate search: apply shapes first, then line shapes.
+template <typename VT>
+static int tune_iters(asam_sound_ctx* c, int prec, int iters, double* ms) {
+  VT *x, *r, *rt, *p, *ph, *v, *rh, *t;
+  if (prec) {
+    x = (VT*)c->d32; r = (VT*)c->r32; rt = (VT*)c->rt32; p = (VT*)c->p32; ph = (VT*)c->ph32;
+    v = (VT*)c->v32; rh = (VT*)c->rh32; t = (VT*)c->t32;
+  } else {
+    x = (VT*)c->xd; r = (VT*)c->r; rt = (VT*)c->rt; p = (VT*)c->p; ph = (VT*)c->ph;
+    v = (VT*)c->v; rh = (VT*)c->rh; t = (VT*)c->t;
+  }
+  launch_fill(c->stream, c->n, prec, x, 0.0);
+  launch_fill(c->stream, c->n, prec, r, 1.0);
+  launch_fill(c->stream, c->n, prec, rt, 1.0);
+  KState* hs = c->st_host;
+  memset(hs, 0, sizeof(KState));
+  hs->iter = 1; hs->maxiter = 1 << 30; hs->tol = 0.0; hs->rho = 1.0; hs->rnorm0 = 1.0;
+  int rc = upload_state(c);
+  if (rc) return rc;
+  Enq e(c, 0);
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int k = 0; k < iters; ++k) bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  HIPCHK(c, hipGetLastError());
+  float tm = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&tm, c->ev0, c->ev1));
+  *ms = (double)tm / iters;
+  return 0;
+}
+
+extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tuning* best,
+                                   double* ms) {
+  if (!c || reps < 1 || reps > 1000) return ASAM_E_INVALID;
+  if (!c->ready) return fail(c, ASAM_E_STATE, "autotune before setup");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int prec = c->ctl.precision == ASAM_PREC_MIXED ? 1 : 0;
+  auto measure = [&](double* out) -> int {
+    double m = 0.0;
+    int rc = prec ? tune_iters<float>(c, 1, 2, &m) : tune_iters<double>(c, 0, 2, &m);  // warm
+    if (rc) return rc;
+    rc = prec ? tune_iters<float>(c, 1, 2 * reps, &m) : tune_iters<double>(c, 0, 2 * reps, &m);
+    *out = m;
+    return rc;
+  };
+  const Tuning t0 = c->g.tune;
+  Tuning bt = t0;
+  double best_ms = 1e30;
+  const int variants[] = {0, 1}, bxs[] = {64, 128, 256}, xcds[] = {1, 0},
+            wgs[] = {2048, 4096, 8192, 16384};
+  for (int var : variants)
+    for (int bx : bxs)
+      for (int xcd : xcds)
+        for (int wg : wgs) {
+          c->g.tune = bt;
+          c->g.tune.apply_variant = var; c->g.tune.apply_bx = bx;
+          c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
+          if (apply_blocks(c->g) > kRedBlocks) continue;
+          double m = 0;
+          int rc = measure(&m);
+          if (rc) { c->g.tune = t0; return rc; }
+          if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+        }
+  const int windows[] = {1, 4, 8}, ldss[] = {1, 0};
+  const Tuning ba = bt;
+  for (int w : windows)
+    for (int l : ldss) {
+      c->g.tune = ba;
+      c->g.tune.line_window = w; c->g.tune.line_lds = l;
+      double m = 0;
+      int rc = measure(&m);
+      if (rc) { c->g.tune = t0; return rc; }
+      if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+    }
+  c->g.tune = bt;
+  memset(c->st_host, 0, sizeof(KState));
+  if (best) asam_sound_get_tuning(c, best);
+  if (ms) { ms[0] = best_ms; ms[1] = 0.0; }
+  return 0;
+}
+
 extern "C" int asam_sound_profile_enable(asam_sound_ctx* c, int on) {
   if (!c) return ASAM_E_INVALID;
   c->prof_on = on ? 1 : 0;

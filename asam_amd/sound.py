@@ -118,6 +118,7 @@ def load(path: str = LIB_PATH):
         "asam_sound_time_copy": (i, [_vp, C.c_size_t, i, dp]),
         "asam_sound_set_tuning": (i, [_vp, C.POINTER(Tuning)]),
         "asam_sound_profile_read": (i, [_vp, dp, dp, ip]),
+        "asam_sound_autotune": (i, [_vp, i, C.POINTER(Tuning), dp]),
         "asam_sound_rhs": (i, [_vp] + [dp] * 6),
         "asam_sound_rhs_device": (i, [_vp] * 7),
         "asam_sound_update": (i, [_vp] + [dp] * 6),
@@ -310,6 +311,14 @@ class SoundSolver:
         else:
             buf.upload(_c(a))
         return buf
+
+    def autotune(self, reps: int = 5):
+        """Pick the fastest launch shapes for the current precision by timing whole BiCGStab
+        iterations; returns (tuning dict, [ms per iteration, 0])."""
+        t = Tuning()
+        ms = (C.c_double * 2)()
+        self._chk(self.L.asam_sound_autotune(self.h, reps, C.byref(t), ms))
+        return t.as_dict(), list(ms)
 
     # ------------------------------------------------------ either side of the solve
     def Rhs(self, U, V, W, incr_rt, incr_r=None):

@@ -350,11 +350,13 @@ def main():
     # with every kernel launch bracketed by HIP events (per-kernel durations and bytes for
     # the roofline).  Both runs issue identical launch sequences.
     s.set_control(precision="FP64")
+    tune64, tms64 = s.autotune(5)
     t64, infos64, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
     t64_i, _, prof64 = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
     x64 = xbuf.download()
 
     s.set_control(precision="MIXED")
+    tunemx, tmsmx = s.autotune(5)
     tmx, infosmx, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
     tmx_i, _, profmx = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
     xmx = xbuf.download()
@@ -398,6 +400,7 @@ def main():
                   "instrumented_ms_per_step": round(tmx_i * 1e3, 4)},
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         "projection": line_proj,
+        "autotuned": {"fp64": tune64, "mixed": tunemx},
         "kernel_timing": "HIP events around every launch of the instrumented replay of the timed solves",
         "kernels_mixed": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
                               "launches_per_solve": v["launches"] / a.steps,

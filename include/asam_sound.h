@@ -106,6 +106,14 @@ int asam_sound_default_control(asam_sound_control *ctl);
 int asam_sound_set_control(asam_sound_ctx *ctx, const asam_sound_control *ctl);
 int asam_sound_get_tuning(const asam_sound_ctx *ctx, asam_sound_tuning *t);
 int asam_sound_set_tuning(asam_sound_ctx *ctx, const asam_sound_tuning *t);
+/* Launch-shape autotuner: for every candidate shape of the stencil apply (variant, width,
+ * XCD order, workgroup count), then of the line solve (window, LDS staging), times whole
+ * BiCGStab iterations (all kernels and finalizes in solve order) on this context's operator
+ * in the precision the control selects (FP32 kernels for ASAM_PREC_MIXED), 2*reps
+ * iterations each, and keeps the fastest.  best and ms[2] (ms per iteration of the chosen
+ * shapes; ms[1] = 0) may be NULL.  Call after set-up; it uses the solve's work vectors and
+ * changes later results only in the summation order of reductions. */
+int asam_sound_autotune(asam_sound_ctx *ctx, int reps, asam_sound_tuning *best, double *ms);
 /* Last error message of this context (static storage of the context). */
 const char *asam_sound_last_error(const asam_sound_ctx *ctx);
 

@@ -214,3 +214,23 @@ def test_device_dot_matches_fsum(sound_mod):
         print(f"n={n}: dot {d1:.17e} fsum {exact:.17e} err {abs(d1 - exact):.2e} bound {bound:.2e}")
         assert d1 == d2
         assert abs(d1 - exact) <= bound
+
+
+@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+def test_autotuned_shapes_keep_parity(precision, oracle_mod, sound_mod):
+    """The autotuner only changes launch shapes: the FP64 solve after it still matches the
+    oracle (1e-10, iterations +-1) and the mixed solve still reaches 1e-8."""
+    p = W.config(4, np1=2, scale=0.5)
+    orc = oracle_mod.Oracle(p)
+    s = sound_mod.SoundSolver.from_problem(p)
+    s.set_control(precision=precision)
+    t, ms = s.autotune(2)
+    print(f"{precision}: autotuned {t}, ms per iteration {ms[0]:.4f}")
+    assert ms[0] > 0
+    x, mi, tl, info = s.SolveSound(np.zeros(p.n), p.b, 200, 1e-8)
+    if precision == "FP64":
+        xo, mo, to, so = orc.bicgstab(p.b, 200, 1e-8)
+        assert rel(x, xo) < SOL_TOL and abs(abs(mi) - abs(mo)) <= 1
+    else:
+        rr = np.linalg.norm(p.b - orc.matvec1(x)) / np.linalg.norm(p.b)
+        assert rr <= 1.0001e-8
