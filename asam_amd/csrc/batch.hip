@@ -249,6 +249,18 @@ __global__ void k_ens_init(KState* st, int nmem, double tol, double inner_tol, i
   for (int q = 0; q < 16; ++q) s->hist[q] = 0;
 }
 
+// Per-member state for timing iterations: active, first iteration, rho = 1, never exits.
+__global__ void k_ens_tune_init(KState* st, int nmem) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nmem) return;
+  KState* s = st + m;
+  s->rho = 1.0; s->rho_old = 1.0; s->alpha = 0.0; s->omega = 0.0; s->beta = 0.0;
+  s->rnorm0 = 1.0; s->rnorm = 1.0; s->tol = 0.0;
+  s->iter = 1; s->maxiter = 1 << 30; s->done = 0; s->status = 1;
+  s->half = 0; s->result = 0; s->first = 0; s->pad = 0;
+  s->outer_done = 0;
+}
+
 // One workgroup per member: sums the member's partials of each quantity in a fixed order
 // (quantity q from producer (srcs >> 2q) & 3), then runs the scalar tail on its state.
 __global__ __launch_bounds__(256) void k_ens_finalize(int kind, int nq, int srcs,
@@ -992,79 +1004,82 @@ int time_launches(asam_ens_ctx* c, int reps, F&& launch, double* ms) {
   return 0;
 }
 
+// Objective: device time of whole FP32 inner BiCGStab iterations over every member
+// (all kernels and finalizes in solve order), so each kernel runs in the cache context it
+// has in a solve.  State: every member active, r = r~ = 1, x = 0, rho = 1, tol = 0.
+template <int NP>
+int time_inner(asam_ens_ctx* c, int iters, double* ms) {
+  launch_fill(c->stream, c->n, 1, c->r, 1.0);
+  launch_fill(c->stream, c->n, 1, c->rt, 1.0);
+  launch_fill(c->stream, c->n, 1, c->d, 0.0);
+  hipLaunchKernelGGL(k_ens_tune_init, dim3((c->nmem + 63) / 64), dim3(64), 0, c->stream, c->st,
+                     c->nmem);
+  EHIP(c, hipEventRecord(c->ev0, c->stream));
+  for (int i = 0; i < iters; ++i) inner_iteration<NP>(c);
+  EHIP(c, hipEventRecord(c->ev1, c->stream));
+  EHIP(c, hipEventSynchronize(c->ev1));
+  EHIP(c, hipGetLastError());
+  float t = 0.f;
+  EHIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = (double)t / iters;
+  return 0;
+}
+
+// Coordinate search over the launch shapes: apply, then line, then update.
 template <int NP>
 int autotune_np(asam_ens_ctx* c, int reps, double* best_ms) {
-  // Kernels run on the live vectors with every member active and alpha = omega = 0 (the
-  // state k_ens_init leaves), so each launch does its full work on finite data.
-  auto reset = [&] {
-    hipLaunchKernelGGL(k_ens_init, dim3((c->nmem + 63) / 64), dim3(64), 0, c->stream, c->st,
-                       c->nmem, 1.0, 1.0, 1 << 30);
+  auto measure = [&](double* out) -> int {
+    double m = 0.0;
+    int rc = build_tables(c);
+    if (rc) return rc;
+    rc = time_inner<NP>(c, 1, &m);  // warm
+    if (rc) return rc;
+    return time_inner<NP>(c, reps, out);
   };
-  asam_ens_tuning t = c->tune;
+  asam_ens_tuning bt = c->tune;
+  double bms = 1e30;
   int rc;
-  // stencil apply: tile width x z-split x persistence
   {
-    double bms = 1e30;
-    asam_ens_tuning bt = t;
     const int bxs[] = {64, 128, 256}, zss[] = {1, 2, 4}, pfs[] = {0, 2, 8};
+    const asam_ens_tuning t0 = bt;
     for (int bx : bxs)
       for (int zs : zss)
         for (int pf : pfs) {
-          c->tune = t;
+          c->tune = t0;
           c->tune.apply_bx = bx; c->tune.apply_zsplit = zs; c->tune.apply_pf = pf;
-          if ((rc = build_tables(c))) return rc;
-          reset();
           double ms = 0;
-          rc = time_launches(c, reps, [&] { apply_inner<NP, AF_YY | AF_YU>(c, c->rh, c->t, c->r); },
-                             &ms);
-          if (rc) return rc;
+          if ((rc = measure(&ms))) return rc;
           if (ms < bms) { bms = ms; bt = c->tune; }
         }
-    t = bt;
-    best_ms[0] = bms;
   }
-  // line solve: window x LDS staging x persistence
   {
-    double bms = 1e30;
-    asam_ens_tuning bt = t;
     const int ws[] = {4, 8}, ls[] = {0, 1}, pfs[] = {0, 4};
-    c->tune = t;
-    if ((rc = build_tables(c))) return rc;
+    const asam_ens_tuning t0 = bt;
     for (int w : ws)
       for (int l : ls)
         for (int pf : pfs) {
+          c->tune = t0;
           c->tune.line_window = w; c->tune.line_lds = l; c->tune.line_pf = pf;
-          reset();
           double ms = 0;
-          rc = time_launches(
-              c, reps, [&] { line_inner<NP, LINE_BICG_S>(c, nullptr, c->r, c->v, c->rh, 2); },
-              &ms);
-          if (rc) return rc;
+          if ((rc = measure(&ms))) return rc;
           if (ms < bms) { bms = ms; bt = c->tune; }
         }
-    t = bt;
-    best_ms[1] = bms;
   }
-  // update: chunk x persistence
   {
-    double bms = 1e30;
-    asam_ens_tuning bt = t;
     const int chs[] = {2048, 8192, 32768}, pfs[] = {0, 4};
+    const asam_ens_tuning t0 = bt;
     for (int ch : chs)
       for (int pf : pfs) {
-        c->tune = t;
+        c->tune = t0;
         c->tune.blas_chunk = ch; c->tune.blas_pf = pf;
-        if ((rc = build_tables(c))) return rc;
-        reset();
         double ms = 0;
-        rc = time_launches(c, reps, [&] { update_inner(c); }, &ms);
-        if (rc) return rc;
+        if ((rc = measure(&ms))) return rc;
         if (ms < bms) { bms = ms; bt = c->tune; }
       }
-    t = bt;
-    best_ms[2] = bms;
   }
-  c->tune = t;
+  c->tune = bt;
+  best_ms[0] = bms;
+  best_ms[1] = best_ms[2] = 0.0;
   return build_tables(c);
 }
 

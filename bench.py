@@ -197,16 +197,16 @@ def ensemble_bench(a):
     E.close()
     del bbuf, xbuf
     print(f"[ensemble] batched solve {t_ens * 1e3:.1f} ms", file=sys.stderr, flush=True)
-    # per-launch algorithmic bytes of the tuned kernels (all members active)
+    # one FP32 inner iteration with every member active (the tuner's objective): line
+    # p-step (5 vectors) + apply v (u) + line s-step (4 vectors) + apply t (u) + update
     np1 = a.np1
     coefw = 5 if np1 == 1 else 9
-    bytes_cell = {"apply_fp32": coefw * 4 + np1 * 12, "line_fp32": 3 * np1 * np1 * 4 + np1 * 16,
-                  "update_fp32": 8 * np1 * 4}
-    kern = {}
-    for (k, bc), ms in zip(bytes_cell.items(), tms):
-        gbs = bc * cells / (ms * 1e-3) / 1e9
-        kern[k] = {"avg_launch_us": round(ms * 1e3, 1), "bytes_per_cell": bc,
-                   "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    fac = 3 * np1 * np1 * 4
+    it_bytes = cells * ((fac + np1 * 4 * 5) + (coefw * 4 + np1 * 12) + (fac + np1 * 4 * 4)
+                        + (coefw * 4 + np1 * 12) + 8 * np1 * 4)
+    gbs = it_bytes / (tms[0] * 1e-3) / 1e9
+    kern = {"inner_iteration": {"ms": round(tms[0], 4), "bytes": int(it_bytes),
+                                "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}}
     # sequential baselines: one drop-in context per member, solves timed on resident inputs
     t64 = tmx = 0.0
     worst_res = worst_diff = 0.0

@@ -84,10 +84,12 @@ int asam_ens_solve_device(asam_ens_ctx *ctx, double *x, const double *b, double 
 
 int asam_ens_get_tuning(const asam_ens_ctx *ctx, asam_ens_tuning *t);
 int asam_ens_set_tuning(asam_ens_ctx *ctx, const asam_ens_tuning *t);
-/* Times every candidate launch shape of the apply, line and update kernels on the
- * ensemble's own data (reps launches each, HIP events), keeps the fastest of each and
- * returns the choice; ms[3] (may be NULL): average ms per launch of the chosen apply,
- * line and update shapes.  Requires every member set up. */
+/* Launch-shape autotuner: for each candidate shape of the apply kernel, then of the line
+ * solve, then of the update kernels (coordinate search), times whole FP32 inner BiCGStab
+ * iterations over every member (all kernels and finalizes in solve order, reps iterations
+ * each) and keeps the fastest.  ms[3] (may be NULL): ms per iteration of the chosen shapes
+ * in ms[0], ms[1] = ms[2] = 0.  Requires every member set up; uses the solve's work
+ * vectors. */
 int asam_ens_autotune(asam_ens_ctx *ctx, int reps, asam_ens_tuning *best, double *ms);
 int asam_ens_get_stats(const asam_ens_ctx *ctx, asam_ens_stats *stats);
 

@@ -75,8 +75,8 @@ def test_autotuned_shapes_solve_correctly(ens_mod, oracle_mod):
     ps = members(2, n=4, seed=11)
     E = ens_mod.Ensemble(ps)
     t, ms = E.autotune(reps=2)
-    print(f"autotuned {t}, ms apply/line/update {ms}")
-    assert all(v > 0 for v in ms)
+    print(f"autotuned {t}, ms per inner iteration {ms[0]:.4f}")
+    assert ms[0] > 0
     x, infos = E.solve([p.b for p in ps], tol=1e-8)
     for p, xm, inf in zip(ps, E.split(x), infos):
         rr = rel_residual(oracle_mod.Oracle(p), xm, p.b)
