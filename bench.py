@@ -148,14 +148,17 @@ def cpu_baseline(p, tol, seconds):
                       f"median {med:.2f} s"}, x, mi, orc
 
 
-def hbm_roofline(np1):
-    """Apply/line kernel bandwidth at 512x512x96 (working set well beyond the 256 MB
-    Infinity Cache), FP32 and FP64, isolated launches timed with HIP events."""
+def hbm_roofline(np1, tol, steps, with_cpu):
+    """The 512x512x96 system (25.2 M cells, working set well beyond the 256 MB Infinity
+    Cache): apply/line kernel bandwidth (isolated launches, HIP events, FP32 and FP64), the
+    FP32-inner / FP64-refined and plain FP64 solve times (each precision autotuned), and the
+    mixed solution's error against the FP64 GPU solve and against the FP64 oracle."""
     from asam_amd import sound, workload
     p = workload.config(3, np1=np1)
     s = sound.SoundSolver.from_problem(p)
     N = p.grid.ncell
     out = {}
+    # isolated kernels first, with the default launch shapes
     for which, name in ((0, "apply"), (1, "line")):
         for fp32 in (True, False):
             ms = s.time_kernel(which, fp32, reps=20)
@@ -165,6 +168,37 @@ def hbm_roofline(np1):
             out[f"{name}_{'fp32' if fp32 else 'fp64'}"] = {
                 "avg_launch_us": round(ms * 1e3, 1), "achieved_GBs": round(gbs, 1),
                 "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    xbuf, bbuf = s.vec(), s.vec(p.b)
+    solves = {}
+    for prec in ("FP64", "MIXED"):
+        s.set_control(precision=prec)
+        s.autotune(3)
+        t, infos, _ = timed_solves(s, xbuf, bbuf, steps, 1, tol, False)
+        solves[prec] = (t, infos[-1], xbuf.download())
+    x64, xmx = solves["FP64"][2], solves["MIXED"][2]
+    rbuf = s.vec()
+    s.apply_device(rbuf, xbuf)
+    res = float(np.linalg.norm(p.b - rbuf.download()) / np.linalg.norm(p.b))
+    out["solve"] = {
+        "mixed_ms": round(solves["MIXED"][0] * 1e3, 3),
+        "fp64_ms": round(solves["FP64"][0] * 1e3, 3),
+        "speedup_mixed_vs_fp64": round(solves["FP64"][0] / solves["MIXED"][0], 3),
+        "mixed_inner_iterations": solves["MIXED"][1]["iterations"],
+        "fp64_iterations": solves["FP64"][1]["iterations"],
+        "mixed_true_rel_residual": res,
+        "mixed_rel_diff_vs_fp64_gpu": float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))}
+    if with_cpu:
+        from oracle import pyoracle
+        pyoracle.set_threads(min(os.cpu_count() or 1, 16))
+        orc = pyoracle.Oracle(p)
+        t0 = time.perf_counter()
+        xo, mo, to, so = orc.bicgstab(p.b, 500, tol)
+        out["solve"]["oracle_fp64_s"] = round(time.perf_counter() - t0, 2)
+        out["solve"]["mixed_err_vs_fp64_oracle"] = float(np.linalg.norm(xmx - xo) /
+                                                         np.linalg.norm(xo))
+        out["solve"]["fp64_err_vs_fp64_oracle"] = float(np.linalg.norm(x64 - xo) /
+                                                        np.linalg.norm(xo))
+        del orc
     s.close()
     return out
 
@@ -413,7 +447,8 @@ def main():
     }
     s.close()
     if not a.no_hbm:
-        line["roofline_512x512x96"] = hbm_roofline(a.np1)
+        line["roofline_512x512x96"] = hbm_roofline(a.np1, a.tol, max(2, a.steps // 4),
+                                                   not a.no_cpu)
     if not a.no_cpu:
         cb, xo, mo, orc = cpu_baseline(p, a.tol, a.cpu_seconds)
         line["cpu_baseline"] = cb
