@@ -169,16 +169,20 @@ def cut_geometry(g: Grid, H: float, a: float, xc: float, vf_min: float = 1e-3):
 
 
 def background_state(g: Grid, bubble_amp: float, bubble_xc: float, bubble_yc: float,
-                     bubble_zc: float = 2000.0, bubble_r: float = 2000.0, theta0: float = 300.0):
+                     bubble_zc: float = 2000.0, bubble_r: float = 2000.0, theta0: float = 300.0,
+                     centres=None):
     """Hydrostatic dry atmosphere with a Gaussian warm bubble.
 
     pi(z) = 1 - g z/(Cpd theta0); p = p0 pi^(Cpd/Rd); theta = theta0 + amp exp(-(r/R)^2);
-    T = theta*pi; rho = p/(Rd T); rhotheta = rho*theta.  Cell-centre coordinates.
-    Returns rho, rhotheta, T, each (nz, ny, nx).
+    T = theta*pi; rho = p/(Rd T); rhotheta = rho*theta.  Cell-centre coordinates (uniform
+    spacing, or centres = (x, y, z) arrays).  Returns rho, rhotheta, T, each (nz, ny, nx).
     """
-    xcen = (np.arange(g.nx) + 0.5) * g.dx
-    ycen = (np.arange(g.ny) + 0.5) * g.dy
-    zcen = (np.arange(g.nz) + 0.5) * g.dz
+    if centres is None:
+        xcen = (np.arange(g.nx) + 0.5) * g.dx
+        ycen = (np.arange(g.ny) + 0.5) * g.dy
+        zcen = (np.arange(g.nz) + 0.5) * g.dz
+    else:
+        xcen, ycen, zcen = centres
     Z, Y, X = np.meshgrid(zcen, ycen, xcen, indexing="ij")
     pi = 1.0 - GRAV * Z / (CPD * theta0)
     p = P0 * pi ** (CPD / RD)
@@ -316,3 +320,26 @@ def stage_inputs(prob: SoundProblem, seed: int = 11, wind: float = 10.0, noise: 
     incr_rt = np.where(live, incr_scale * normal(g.ncell, seed + 3).reshape(live.shape), 0.0)
     incr_r = np.where(live, incr_scale * normal(g.ncell, seed + 4).reshape(live.shape), 0.0)
     return U, V, W, incr_rt, incr_r
+
+
+def problem_from_geometry(geo, np1: int = 1, grad_full: int = 1, seed: int = 1,
+                          bubble_amp: float = 2.0, dt: float = 2.0, name: str = ""):
+    """A stage system on a given cut-cell geometry (e.g. read from a .Weight2 file, see
+    asam_amd.weights): the hydrostatic background with a bubble at the horizontal centre,
+    2 km up, and b ~ N(0,1) from `seed` on live cells."""
+    dx, dy, dz = (np.asarray(a, dtype=np.float64) for a in (geo.dx, geo.dy, geo.dz))
+    g = Grid(len(dx), len(dy), len(dz), float(dx.mean()), float(dy.mean()), float(dz.mean()))
+    xe = geo.x0 + np.concatenate([[0.0], np.cumsum(dx)])
+    ye = geo.y0 + np.concatenate([[0.0], np.cumsum(dy)])
+    ze = geo.z0 + np.concatenate([[0.0], np.cumsum(dz)])
+    cen = (0.5 * (xe[1:] + xe[:-1]), 0.5 * (ye[1:] + ye[:-1]), 0.5 * (ze[1:] + ze[:-1]))
+    rho, rhotheta, T = background_state(g, bubble_amp, 0.5 * (xe[0] + xe[-1]),
+                                        0.5 * (ye[0] + ye[-1]), centres=cen)
+    V = np.asarray(geo.VolC, dtype=np.float64)
+    act = (V > 0.0).reshape(-1)
+    b = normal(g.ncell * np1, seed).reshape(-1, np1)
+    b[~act, :] = 0.0
+    return SoundProblem(g, np1, grad_full, GAMMA_ROS * dt, GRAV if np1 == 2 else 0.0,
+                        np.asarray(geo.FU, dtype=np.float64), np.asarray(geo.FV, dtype=np.float64),
+                        np.asarray(geo.FW, dtype=np.float64), V, dx, dy, dz, rho, rhotheta, T,
+                        b.reshape(-1), name)

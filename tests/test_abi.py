@@ -53,7 +53,7 @@ int main(void) {
 def test_no_cpu_fallback_in_product(sound_mod):
     """The product path never imports the oracle: the host mirror fails loudly without
     the HIP library."""
-    for f in ("sound.py", "ensemble.py", "rosenbrock.py", "workload.py"):
+    for f in ("sound.py", "ensemble.py", "rosenbrock.py", "workload.py", "weights.py"):
         src = open(os.path.join(ROOT, "asam_amd", f)).read()
         assert "import oracle" not in src and "from oracle" not in src, f
         assert "pyoracle" not in src and "liborc" not in src, f
