@@ -343,3 +343,218 @@ def problem_from_geometry(geo, np1: int = 1, grad_full: int = 1, seed: int = 1,
                         np.asarray(geo.FU, dtype=np.float64), np.asarray(geo.FV, dtype=np.float64),
                         np.asarray(geo.FW, dtype=np.float64), V, dx, dy, dz, rho, rhotheta, T,
                         b.reshape(-1), name)
+
+
+# ---------------------------------------------------------------------------
+# Transport stage system (the second implicit solve of a Rosenbrock stage).
+# ---------------------------------------------------------------------------
+EPS = 1.0e-30  # Kind_Mod Eps
+
+# Operator of one component in MatVec / PSolve (PSolveSp, MatVecMetSp branches,
+# SRC/MODULE/Vector4Cell_Mod.f90:605-644, 694-740).
+TR_MOM, TR_POT, TR_FALL = 0, 1, 2
+
+
+@dataclasses.dataclass
+class TransportProblem:
+    """One block's transport stage system in the reference's SpDiag form.
+
+    tmom, tpot: Val of JacTMom / JacTPot, shape (7, m) -- numpy C order of the Fortran
+    Val(m, 7), offsets -(nx+2)(ny+2), -(nx+2), -1, 0, 1, nx+2, (nx+2)(ny+2), main diagonal 4
+    (SpDiagAllocate, SRC/MODULE/Vector4Cell_Mod.f90:1020-1046); fall: JacFall Val (2, m),
+    offsets 0, (nx+2)(ny+2), or None; lu: per-cell LU factors (nc*nc, ncell) of
+    I - beta0*dt*J_chem (GefaMatrix), entry (ic,kc) in row ic*nc+kc, or None; b: (nc, m)
+    haloed right-hand sides; op_mv / op_ps: per-component operator (TR_*); alpha: ScaleMat2."""
+    nx: int
+    ny: int
+    nz: int
+    ncomp: int
+    iterg: int
+    beta0_dt: float
+    tmom: np.ndarray
+    tpot: np.ndarray
+    fall: np.ndarray | None
+    lu: np.ndarray | None
+    op_mv: np.ndarray
+    op_ps: np.ndarray
+    alpha: np.ndarray
+    b: np.ndarray
+    name: str = ""
+
+    @property
+    def m(self) -> int:
+        return (self.nx + 2) * (self.ny + 2) * (self.nz + 2)
+
+    @property
+    def ncell(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def interior(self, v: np.ndarray) -> np.ndarray:
+        """View (nc, nz, ny, nx) of the interior cells of a haloed vector (nc, m)."""
+        return v.reshape(self.ncomp, self.nz + 2, self.ny + 2, self.nx + 2)[:, 1:-1, 1:-1, 1:-1]
+
+
+def _halo_index(nx, ny, nz):
+    """Linear haloed index of every interior cell, shape (nz, ny, nx)."""
+    X, Y = nx + 2, ny + 2
+    k, j, i = np.meshgrid(np.arange(1, nz + 1), np.arange(1, ny + 1), np.arange(1, nx + 1),
+                          indexing="ij")
+    return i + X * (j + Y * k)
+
+
+def transport_jacobian(prob: SoundProblem, U, V, W, kdiff: float = 0.0, fall: bool = False):
+    """Unscaled transport Jacobian J (7, m) of one tracer, and JacFall (2, m) if `fall`.
+
+    Upwind advection as JacAdvectionCompute (SRC/ADVECT/Advection_Mod.f90:2247-2330):
+    for the interior face between cells L and R with flux velocity u, area F,
+    VL/VR = VolC + Eps, column L gets +(u+|u|)/2 F/VR/(rho_L+Eps) on the diagonal towards
+    R and -(u+|u|)/2 F/VL/(rho_L+Eps) on its own, column R the mirrored (|u|-u)/2 terms.
+    Diffusion as JacDiffusionCompute (SRC/ADVECT/Diffusion_Mod.f90:1504-1708) on interior
+    faces with DH = kdiff*rho and the arithmetic face mean DF (ADVECT/Operator_Mod.f90:49-53):
+    Temp = 2 DF F^2/(VL+VR+Eps), +Temp/(VR+Eps)/(rho_L+Eps) towards R, -Temp/(VL+Eps)/...
+    on the diagonal.  Sedimentation (JacFall) on vertical faces: column of the upper cell,
+    -F/VL/(rho_U+Eps) into the cell below and +F/VR/(rho_U+Eps) on its own diagonal.
+    Contributions accumulate in the reference's loop order (x, y, z faces; per face the
+    left cell first)."""
+    g = prob.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    X, Y = nx + 2, ny + 2
+    m = X * Y * (nz + 2)
+    J = np.zeros((7, m))
+    JF = np.zeros((2, m)) if fall else None
+    hid = _halo_index(nx, ny, nz)
+    vol = prob.VolC
+    rho = prob.rho
+    # (axis of the cell arrays, face slice of the velocity/area, diag towards +, diag towards -)
+    dirs = ((2, U[:, :, 1:nx], prob.FU[:, :, 1:nx], 2, 4),
+            (1, V[:, 1:ny, :], prob.FV[:, 1:ny, :], 1, 5),
+            (0, W[1:nz], prob.FW[1:nz], 0, 6))
+
+    def lo(a, ax):
+        s = [slice(None)] * 3
+        s[ax] = slice(0, -1)
+        return a[tuple(s)]
+
+    def hi(a, ax):
+        s = [slice(None)] * 3
+        s[ax] = slice(1, None)
+        return a[tuple(s)]
+
+    def run(term):
+        for ax, u, F, dplus, dminus in dirs:
+            VL, VR = lo(vol, ax) + EPS, hi(vol, ax) + EPS
+            rL, rR = lo(rho, ax), hi(rho, ax)
+            iL, iR = lo(hid, ax), hi(hid, ax)
+            toR, diagL, toL, diagR = term(u, F, VL, VR, rL, rR, lo(vol, ax), hi(vol, ax))
+            # loop order per face: left column, then right column; a cell is the right
+            # cell of the face before it first, so its diagonal takes that term first
+            dR = np.zeros_like(vol)
+            dL = np.zeros_like(vol)
+            s = [slice(None)] * 3
+            s[ax] = slice(1, None)
+            dR[tuple(s)] = diagR
+            s[ax] = slice(0, -1)
+            dL[tuple(s)] = diagL
+            J[dplus][iL.ravel()] += toR.ravel()
+            J[dminus][iR.ravel()] += toL.ravel()
+            J[3][hid.ravel()] = (J[3][hid.ravel()] - dR.ravel()) - dL.ravel()
+
+    def adv(u, F, VL, VR, rL, rR, vL, vR):
+        up = 0.5 * (np.abs(u) + u)
+        um = 0.5 * (np.abs(u) - u)
+        return (up * F / VR / (rL + EPS), up * F / VL / (rL + EPS),
+                um * F / VL / (rR + EPS), um * F / VR / (rR + EPS))
+
+    def diff(u, F, VL, VR, rL, rR, vL, vR):
+        DL, DR = kdiff * rL, kdiff * rR
+        DF = (DL * vR + DR * vL) / (vL + vR + EPS)
+        T = 2.0 * DF * F * F / (vL + vR + EPS)
+        return (T / (vR + EPS) / (rL + EPS), T / (vL + EPS) / (rL + EPS),
+                T / (vL + EPS) / (rR + EPS), T / (vR + EPS) / (rR + EPS))
+
+    run(adv)
+    if kdiff > 0.0:
+        run(diff)
+    if fall:
+        F = prob.FW[1:nz]
+        VL, VR = vol[:-1] + EPS, vol[1:] + EPS
+        rU = rho[1:]
+        iU = hid[1:].ravel()
+        JF[1][iU] -= (F / VL / (rU + EPS)).ravel()
+        JF[0][iU] += (F / VR / (rU + EPS)).ravel()
+    return J, JF
+
+
+def transport_scale(J: np.ndarray, JF, beta0_dt: float):
+    """I - beta0*dt*J on the SpDiag values (JacScaleCompute, SRC/ADVECT/Function_Mod.f90
+    :1857-1869; closed walls add nothing because no wall face is assembled)."""
+    A = -(beta0_dt * J)
+    A[3] = 1.0 - beta0_dt * J[3]
+    AF = None if JF is None else -(beta0_dt * JF)
+    return A, AF
+
+
+def transport_problem(prob: SoundProblem, ncomp: int = 4, seed: int = 31, wind: float = 10.0,
+                      noise: float = 1.0, kdiff_mom: float = 50.0, kdiff_pot: float = 20.0,
+                      chem: bool = True, fall_comp: int | None = None,
+                      fall_speed: float = -2.0, met: bool = False, mom_comps=(),
+                      iterg: int = 5, name: str = "") -> TransportProblem:
+    """The transport stage system of `ncomp` tracers on a sound problem's geometry and
+    state: face velocities from `stage_inputs` (wind + noise), JacTMom / JacTPot with
+    momentum / scalar diffusivities, an optional JacFall for component `fall_comp`
+    (ScaleMat2 = fall_speed < 0), and with `chem` a per-cell linear reaction network
+    (decay rates 1e-3..1e-1 /s on the diagonal, transfer -0.5 of the decay of the species
+    before) LU-factored by GefaMatrix.  BiCGStabChem form (matvec with JacTPot for every
+    component) unless `met`, where the matvec uses each component's own operator.
+    b ~ N(0,1) on live interior cells, 0 on the halo."""
+    g = prob.grid
+    nx, ny, nz = g.nx, g.ny, g.nz
+    U, V, W, _, _ = stage_inputs(prob, seed, wind=wind, noise=noise)
+    Jm, JF = transport_jacobian(prob, U, V, W, kdiff_mom, fall=fall_comp is not None)
+    Jp, _ = transport_jacobian(prob, U, V, W, kdiff_pot)
+    tmom, tfall = transport_scale(Jm, JF, prob.beta0_dt)
+    tpot, _ = transport_scale(Jp, None, prob.beta0_dt)
+    op_ps = np.full(ncomp, TR_POT, dtype=np.int32)
+    for c in mom_comps:
+        op_ps[c] = TR_MOM
+    alpha = np.zeros(ncomp)
+    if fall_comp is not None:
+        op_ps[fall_comp] = TR_FALL
+        alpha[fall_comp] = fall_speed
+    op_mv = op_ps.copy() if met else np.full(ncomp, TR_POT, dtype=np.int32)
+    lu = None
+    ncell = g.ncell
+    if chem:
+        rates = 10.0 ** (-3.0 + 2.0 * uniform01(ncomp, seed + 7))
+        Jc = np.zeros((ncomp * ncomp, ncell))
+        for c in range(ncomp):
+            Jc[c * ncomp + c] = -rates[c]
+            if c > 0:
+                Jc[c * ncomp + c - 1] = 0.5 * rates[c - 1]
+        # spatial modulation so that every cell has its own factors
+        mod = 1.0 + 0.5 * uniform01(ncell, seed + 8)
+        Jc *= mod[None, :]
+        lu = gefa(Jc, ncomp, prob.beta0_dt)
+    live = (prob.VolC > 0.0).reshape(-1)
+    b = np.zeros((ncomp, (nx + 2) * (ny + 2) * (nz + 2)))
+    hid = _halo_index(nx, ny, nz).ravel()
+    for c in range(ncomp):
+        bc = normal(ncell, seed + 100 + c)
+        bc[~live] = 0.0
+        b[c, hid] = bc
+    return TransportProblem(nx, ny, nz, ncomp, iterg, prob.beta0_dt, tmom, tpot, tfall, lu,
+                            op_mv.astype(np.int32), op_ps, alpha, b, name or prob.name)
+
+
+def gefa(J: np.ndarray, ncomp: int, fac: float) -> np.ndarray:
+    """GefaMatrix (SRC/MODULE/Vector4Cell_Mod.f90:1404-1429) on per-cell Jacobians J
+    (nc*nc, ncell): LU = I - fac*J, Doolittle without pivoting, in the reference's order."""
+    LU = -(fac * J)
+    for c in range(ncomp):
+        LU[c * ncomp + c] = 1.0 + LU[c * ncomp + c]
+    for k in range(ncomp):
+        for i in range(k + 1, ncomp):
+            LU[i * ncomp + k] = LU[i * ncomp + k] / LU[k * ncomp + k]
+            for j in range(k + 1, ncomp):
+                LU[i * ncomp + j] = LU[i * ncomp + j] - LU[i * ncomp + k] * LU[k * ncomp + j]
+    return LU
