@@ -62,11 +62,12 @@ _lib = None
 
 
 def exported_names_from_header(path: str = None):
-    """Function names declared in include/asam_sound.h and include/asam_ensemble.h (or
-    in the one header named)."""
+    """Function names declared in include/asam_sound.h, asam_ensemble.h and
+    asam_transport.h (or in the one header named)."""
     import re
-    paths = [path] if path else [HEADER, os.path.join(os.path.dirname(HEADER),
-                                                      "asam_ensemble.h")]
+    inc = os.path.dirname(HEADER)
+    paths = [path] if path else [HEADER, os.path.join(inc, "asam_ensemble.h"),
+                                 os.path.join(inc, "asam_transport.h")]
     names = set()
     for pth in paths:
         txt = open(pth).read()
