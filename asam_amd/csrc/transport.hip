@@ -201,6 +201,13 @@ struct Pre {
 // One forward sweep (L+D) x_new = b - U x_old; sweep 0 starts from x_old = 0 (the first
 // forward substitution of GaussSeidel_SpDiag :204-212).  Order of the terms of a row as
 // in the reference: b, -U (+1, +X, +XY, then JacFall's +XY), -L (-XY, -X, -1), / diag.
+//
+// Step T of the workgroup is step s = T - j of wave j.  The coefficients, b and the old
+// neighbours of step s are loaded D steps ahead into ring slot T mod D; the loop is
+// unrolled by D so every slot index is a constant (no register moves, so a step waits
+// only for the loads issued D steps before).  New values are stored with agent-scope
+// (write-through) stores, so the L2 holds no dirty lines of the vector and the release
+// that publishes a block's progress has little to write back.
 template <int M, int D>
 __global__ __launch_bounds__(512) void k_gs(GsArgs a, Ops ops) {
 #pragma clang fp contract(off)
@@ -227,25 +234,30 @@ __global__ __launch_bounds__(512) void k_gs(GsArgs a, Ops ops) {
   const long rowbase = (long)(row_ok ? iy : 0) * SZ;
   const bool upd = a.sweep > 0;
 
-  Pre<M> pre[D];
+  // Loads are unconditional and branch-free: the slot index is clamped into the row and
+  // the skewed vector carries S*Z + Z + 1 entries of padding, so every address is valid.
+  // Values of slots or neighbours that do not exist are never used: the same conditions
+  // guard their use.  With no branches around the loads the wait counts stay exact and a
+  // step waits only for the loads issued D steps before it.
   auto load = [&](int sl, Pre<M>& p) {
+    const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
 #pragma unroll
     for (int mm = 0; mm < M; ++mm) {
-      const int iz = lane + 64 * mm, ix = sl - iz;
-      const bool ok = row_ok && sl < S && iz < Z && ix >= 0 && ix < X;
-      const long k = rowbase + (long)sl * Z + iz;
+      const int iz = lane + 64 * mm;
+      const long k = rowbase + (long)slc * Z + (iz < Z ? iz : Z - 1);
 #pragma unroll
-      for (int q = 0; q < 7; ++q) p.w[mm][q] = ok ? W[q * sks + k] : 0.0;
-      p.b[mm] = ok ? bs[k] : 0.0;
-      p.xo[mm][0] = (ok && upd && ix + 1 < X) ? xs[k + Z] : 0.0;
-      p.xo[mm][1] = (ok && upd && iy + 1 < Y) ? xs[k + SZ] : 0.0;
-      p.xo[mm][2] = (ok && upd && iz + 1 < Z) ? xs[k + Z + 1] : 0.0;
-      p.f[mm][0] = (ok && fall) ? a.wf[k] : 0.0;
-      p.f[mm][1] = (ok && fall) ? a.wf[sks + k] : 0.0;
+      for (int q = 0; q < 7; ++q) p.w[mm][q] = W[q * sks + k];
+      p.b[mm] = bs[k];
+      p.xo[mm][0] = xs[k + Z];
+      p.xo[mm][1] = xs[k + SZ];
+      p.xo[mm][2] = xs[k + Z + 1];
+      p.f[mm][0] = a.wf[k];
+      p.f[mm][1] = a.wf[sks + k];
     }
   };
+  Pre<M> pre[D];
 #pragma unroll
-  for (int q = 0; q < D; ++q) load(q, pre[q]);
+  for (int ph = 0; ph < D; ++ph) load(ph - j, pre[ph]);  // slot of step T = ph
 
   double prev[M];
 #pragma unroll
@@ -253,82 +265,85 @@ __global__ __launch_bounds__(512) void k_gs(GsArgs a, Ops ops) {
   int seen = 0;
   bool broken = false;
   const int steps = S + R - 1;
-  for (int T = 0; T < steps; ++T) {
-    const int s = T - j;
-    const bool active = row_ok && s >= 0 && s < S;
-    double up[M];
+  for (int T0 = 0; T0 < steps; T0 += D) {
 #pragma unroll
-    for (int mm = 0; mm < M; ++mm) up[mm] = __shfl(prev[mm], (lane + 63) & 63);
-    // first row of a block after the first: row iy-1 of the previous block must have
-    // finished step s
-    if (j == 0 && g > 0 && active && !broken && seen < s + 1) {
-      int v = 0;
-      if (lane == 0) {
-        int n = 0;
-        while ((v = __hip_atomic_load(flag + g - 1, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT)) < s + 1) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++n > kSpinLimit) {
-            __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = -1;
-            break;
+    for (int ph = 0; ph < D; ++ph) {
+      const int T = T0 + ph;
+      if (T >= steps) break;  // uniform across the workgroup
+      const int s = T - j;
+      const bool active = row_ok && s >= 0 && s < S;
+      double up[M];
+#pragma unroll
+      for (int mm = 0; mm < M; ++mm) up[mm] = __shfl(prev[mm], (lane + 63) & 63);
+      // first row of a block after the first: row iy-1 of the previous block must have
+      // finished step s
+      if (j == 0 && g > 0 && active && !broken && seen < s + 1) {
+        int v = 0;
+        if (lane == 0) {
+          int n = 0;
+          while ((v = __hip_atomic_load(flag + g - 1, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT)) < s + 1) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++n > kSpinLimit) {
+              __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              v = -1;
+              break;
+            }
           }
         }
+        v = __shfl(v, 0);
+        if (v < 0) {
+          broken = true;
+        } else {
+          seen = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
-      v = __shfl(v, 0);
-      if (v < 0) {
-        broken = true;
-      } else {
-        seen = v;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    }
-    double out[M];
-    const int pb = ((T - 1) & 1) * R * Z, cb = (T & 1) * R * Z;
+      const int pb = ((T - 1) & 1) * R * Z, cb = (T & 1) * R * Z;
+      const Pre<M>& p = pre[ph];
+      double out[M];
 #pragma unroll
-    for (int mm = 0; mm < M; ++mm) {
-      out[mm] = 0.0;
-      const int iz = lane + 64 * mm, ix = s - iz;
-      if (active && iz < Z && ix >= 0 && ix < X) {
-        const long k = rowbase + (long)s * Z + iz;
-        const Pre<M>& p = pre[0];
-        double t = p.b[mm];
-        if (upd) {
-          if (ix + 1 < X) t = t - p.w[mm][4] * p.xo[mm][0];
-          if (iy + 1 < Y) t = t - p.w[mm][5] * p.xo[mm][1];
-          if (iz + 1 < Z) {
-            t = t - p.w[mm][6] * p.xo[mm][2];
-            if (fall) t = t - (p.f[mm][1] * alpha) * p.xo[mm][2];
+      for (int mm = 0; mm < M; ++mm) {
+        out[mm] = 0.0;
+        const int iz = lane + 64 * mm, ix = s - iz;
+        const bool valid = active && iz < Z && ix >= 0 && ix < X;
+        // row iy-1: from the wave above through LDS, or from the previous block
+        double xy_l = 0.0, xy_g = 0.0;
+        if (j > 0 && iz < Z) xy_l = lds[pb + (j - 1) * Z + iz];
+        if (j == 0 && valid && iy > 0) xy_g = xs[rowbase + (long)s * Z + iz - SZ];
+        if (valid) {
+          const long k = rowbase + (long)s * Z + iz;
+          double t = p.b[mm];
+          if (upd) {
+            if (ix + 1 < X) t = t - p.w[mm][4] * p.xo[mm][0];
+            if (iy + 1 < Y) t = t - p.w[mm][5] * p.xo[mm][1];
+            if (iz + 1 < Z) {
+              t = t - p.w[mm][6] * p.xo[mm][2];
+              if (fall) t = t - (p.f[mm][1] * alpha) * p.xo[mm][2];
+            }
           }
+          if (iz > 0) {
+            const double xz = lane > 0 ? up[mm] : up[mm > 0 ? mm - 1 : 0];
+            t = t - p.w[mm][0] * xz;
+          }
+          if (iy > 0) t = t - p.w[mm][1] * (j > 0 ? xy_l : xy_g);
+          if (ix > 0) t = t - p.w[mm][2] * prev[mm];
+          double dg = p.w[mm][3];
+          if (fall) dg = dg + p.f[mm][0] * alpha;
+          out[mm] = t / dg;
+          __hip_atomic_store(xs + k, out[mm], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (iz > 0) {
-          const double xz = lane > 0 ? up[mm] : up[mm > 0 ? mm - 1 : 0];
-          t = t - p.w[mm][0] * xz;
-        }
-        if (iy > 0) {
-          const double xy = j > 0 ? lds[pb + (j - 1) * Z + iz] : xs[k - SZ];
-          t = t - p.w[mm][1] * xy;
-        }
-        if (ix > 0) t = t - p.w[mm][2] * prev[mm];
-        double dg = p.w[mm][3];
-        if (fall) dg = dg + p.f[mm][0] * alpha;
-        out[mm] = t / dg;
-        xs[k] = out[mm];
+        if (iz < Z) lds[cb + j * Z + iz] = out[mm];
+        prev[mm] = out[mm];
       }
-      if (iz < Z) lds[cb + j * Z + iz] = out[mm];
-      prev[mm] = out[mm];
+      load(s + D, pre[ph]);
+      if (j == R - 1 && g + 1 < a.G && active && ((s + 1) % a.publish == 0 || s == S - 1)) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0)
+          __hip_atomic_store(flag + g, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
     }
-    if (active) {
-#pragma unroll
-      for (int q = 0; q + 1 < D; ++q) pre[q] = pre[q + 1];
-      load(s + D, pre[D - 1]);
-    }
-    if (j == R - 1 && active && ((s + 1) % a.publish == 0 || s == S - 1)) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      if (lane == 0)
-        __hip_atomic_store(flag + g, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
   }
 }
 
@@ -503,7 +518,7 @@ extern "C" int asam_tr_create(asam_tr_ctx** out, const asam_tr_grid* g) {
   rc |= tralloc(c, &c->wpot, 7 * (size_t)d.sks);
   rc |= tralloc(c, &c->wf, 2 * (size_t)d.sks);
   rc |= tralloc(c, &c->bs, (size_t)c->nc * d.sks);
-  rc |= tralloc(c, &c->xs, (size_t)c->nc * d.sks);
+  rc |= tralloc(c, &c->xs, (size_t)c->nc * d.sks + (size_t)d.S * d.Z + d.Z + 1);
   double** vecs[] = {&c->vx, &c->vb, &c->vr, &c->vrt, &c->vrh, &c->vp, &c->vph, &c->vt, &c->vv,
                      &c->vh};
   for (double** v : vecs) rc |= tralloc(c, v, V);
@@ -518,7 +533,8 @@ extern "C" int asam_tr_create(asam_tr_ctx** out, const asam_tr_grid* g) {
   if (hipMemsetAsync(c->wmom, 0, 7 * sizeof(double) * d.sks, c->stream) != hipSuccess ||
       hipMemsetAsync(c->wpot, 0, 7 * sizeof(double) * d.sks, c->stream) != hipSuccess ||
       hipMemsetAsync(c->wf, 0, 2 * sizeof(double) * d.sks, c->stream) != hipSuccess ||
-      hipMemsetAsync(c->xs, 0, (size_t)c->nc * sizeof(double) * d.sks, c->stream) != hipSuccess ||
+      hipMemsetAsync(c->xs, 0, sizeof(double) * ((size_t)c->nc * d.sks + (size_t)d.S * d.Z + d.Z + 1),
+                     c->stream) != hipSuccess ||
       hipMemsetAsync(c->bs, 0, (size_t)c->nc * sizeof(double) * d.sks, c->stream) != hipSuccess ||
       hipMemsetAsync(c->sync, 0, sizeof(int) * c->sync_n, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)

@@ -39,6 +39,10 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-hbm", action="store_true", help="skip the 512x512x96 kernel roofline")
+    ap.add_argument("--no-transport", action="store_true",
+                    help="skip the transport stage solve leg")
+    ap.add_argument("--transport-only", action="store_true",
+                    help="measure only the transport stage solve leg and exit")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--kernel-probe", type=int, default=0, metavar="CFG",
                     help="time the copy, apply and line kernels in isolation on config CFG "
@@ -360,6 +364,78 @@ def projection_leg(s, p, steps, tol):
                        "GBs": round(ub / (ms_u * 1e-3) / 1e9, 1)}}
 
 
+def transport_leg(steps, with_cpu, cpu_seconds=15.0, ncomp=4, tol=1e-6, maxiter=30):
+    """The transport stage solve (BiCGStabChemSp, reference MODULE/SpMatrix4CellPar_Mod.f90
+    :140-243) of `ncomp` tracers on the 256x256x64 cut-cell grid: upwind advection and
+    diffusion Jacobian from the stage velocities, a per-cell chemistry LU, x0 = 0 and the
+    reference's BiCGStabTol / BiCGStabMaxIter defaults (Control_Mod.f90:580-581).  Solves
+    on resident device vectors; the Gauss-Seidel sweep's algorithmic bytes per launch are
+    ncomp * m * (7 coefficients + b + old x + new x) words; the oracle solve is timed on
+    the host cores beside it."""
+    from asam_amd import transport, workload
+    p = workload.config(2, np1=1)
+    tp = workload.transport_problem(p, ncomp=ncomp)
+    s = transport.TransportSolver.from_problem(tp)
+    xd, bd = s.device_vector(), s.device_vector().upload(tp.b)
+    zero = np.zeros(tp.ncomp * tp.m)
+    times, infos = [], []
+    for step in range(steps + 1):
+        xd.upload(zero)
+        t0 = time.perf_counter()
+        mi, tl, info = s.solve_device(xd, bd, maxiter, tol)
+        if step:
+            times.append(time.perf_counter() - t0)
+            infos.append((mi, tl, info.as_dict()))
+    x = xd.download().reshape(tp.ncomp, tp.m)
+    ms_sweep = s.time_kernel(0, 10)
+    ms_mv = s.time_kernel(1, 10)
+    ms_ps = s.time_kernel(2, 3)
+    sweep_bytes = tp.ncomp * tp.m * 10 * 8
+    mv_bytes = tp.m * (7 + 3 * tp.ncomp + (tp.ncomp ** 2 if tp.lu is not None else 0)) * 8
+    med = float(np.median(times))
+    out = {"workload": f"256x256x64 cut-cell transport stage solve, {ncomp} tracers "
+                       f"(advection + diffusion, per-cell chemistry LU), BiCGStabChem + "
+                       f"{1 + tp.iterg} forward Gauss-Seidel sweeps, tol {tol:g}",
+           "solves_per_s": round(1.0 / med, 3), "ms_per_solve": round(1e3 * med, 3),
+           "maxiter_out": infos[-1][0], "tol_out": infos[-1][1], "info": infos[-1][2],
+           "tuning": s.get_tuning().as_dict(),
+           "gs_sweep": {"avg_launch_us": round(ms_sweep * 1e3, 1), "bytes": sweep_bytes,
+                        "GBs": round(sweep_bytes / (ms_sweep * 1e-3) / 1e9, 1),
+                        "frac": round(sweep_bytes / (ms_sweep * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "bound": "latency: one lexicographic sweep is a chain of "
+                                 f"{(tp.nx + 2) + (tp.ny + 2) + (tp.nz + 2) - 2} hyperplanes"},
+           "matvec": {"avg_launch_us": round(ms_mv * 1e3, 1), "bytes": mv_bytes,
+                      "GBs": round(mv_bytes / (ms_mv * 1e-3) / 1e9, 1)},
+           "psolve_ms": round(ms_ps, 3)}
+    for v in (xd, bd):
+        v.free()
+    del s
+    if with_cpu:
+        from oracle import pyoracle, pytransport
+        threads = min(os.cpu_count() or 1, 16)
+        pyoracle.set_threads(threads)
+        o = pytransport.TransportOracle(tp)
+        ct = []
+        xo = None
+        while not ct or (sum(ct) < cpu_seconds and len(ct) < 3):
+            t0 = time.perf_counter()
+            xo, mio, tlo, sto = o.bicgstab(tp.b, maxiter, tol)
+            ct.append(time.perf_counter() - t0)
+        cmed = float(np.median(ct))
+        out["cpu_baseline"] = {"value": round(1.0 / cmed, 4), "unit": "solves/s",
+                               "cores": threads, "kind": "port",
+                               "sample": f"{len(ct)} full FP64 transport solves of the same "
+                                         f"system (C restatement; Gauss-Seidel single-threaded "
+                                         f"as in the reference, products and BLAS on {threads} "
+                                         f"threads), median {cmed:.2f} s"}
+        out["oracle_maxiter_out"] = mio
+        inner = tp.interior(x).ravel()
+        innero = tp.interior(xo).ravel()
+        out["err_vs_fp64_oracle"] = float(np.linalg.norm(inner - innero) /
+                                          np.linalg.norm(innero))
+    return out
+
+
 def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -373,6 +449,10 @@ def main():
         return
     if a.ensemble:
         ensemble_bench(a)
+        return
+    if a.transport_only:
+        print(json.dumps({"transport": transport_leg(max(2, a.steps // 4), not a.no_cpu,
+                                                     a.cpu_seconds)}))
         return
     from asam_amd import sound, workload
     p = workload.config(2, np1=a.np1)
@@ -457,6 +537,8 @@ def main():
         line["fp64"]["oracle_maxiter_out"] = mo
     else:
         line["cpu_baseline"] = None
+    if not a.no_transport:
+        line["transport"] = transport_leg(max(2, a.steps // 4), not a.no_cpu, a.cpu_seconds)
     print(json.dumps(line))
 
 
