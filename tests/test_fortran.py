@@ -22,7 +22,7 @@ needs_flang = pytest.mark.skipif(not os.path.exists(FLANG), reason="amdflang not
 
 
 def ensure_built():
-    if not os.path.exists(DRV):
+    if not (os.path.exists(DRV) and os.path.exists(os.path.join(FDIR, "_build", "drv_transport"))):
         subprocess.run(["make", "-C", FDIR], check=True, capture_output=True)
     return DRV
 
@@ -136,3 +136,66 @@ def test_fortran_project_vel_face_matches_oracle(np1, oracle_mod, tmp_path):
         d = np.abs(g_ - o_).max() / max(np.abs(o_).max(), 1e-300)
         print(f"np1={np1} {nm}: rel diff {d:.2e}")
         assert d < 1e-10, nm
+
+
+@needs_flang
+def test_bind_c_transport_types_match_header(sound_mod):
+    import ctypes as C
+    from asam_amd import transport as T
+    src = """program sizes
+  use iso_c_binding
+  use asam_transport_iface
+  type(asam_tr_grid) :: g
+  type(asam_tr_info) :: i
+  print '(2i6)', c_sizeof(g), c_sizeof(i)
+end program
+"""
+    ensure_built()
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "s.f90")
+        open(f, "w").write(src)
+        exe = os.path.join(d, "s")
+        subprocess.run([FLANG, "-I", os.path.join(FDIR, "_build"), f,
+                        os.path.join(FDIR, "_build", "asam_transport_iface.o"),
+                        "-L" + LIBDIR, "-lasam_sound", "-Wl,-rpath," + LIBDIR, "-o", exe],
+                       check=True, capture_output=True)
+        vals = [int(v) for v in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    assert vals == [C.sizeof(T.TrGrid), C.sizeof(T.TrInfo)]
+
+
+@needs_flang
+@pytest.mark.gpu
+@pytest.mark.parametrize("met", [0, 1])
+def test_fortran_transport_matches_oracle(met, oracle_mod, tmp_path):
+    """BiCGStabChemSp / BiCGStabMetSp called from Fortran against the oracle's solve."""
+    from oracle import pytransport
+    ensure_built()
+    drv = os.path.join(FDIR, "_build", "drv_transport")
+    p = W.make_problem(30, 22, 14, 500.0, 500.0, 250.0, H=700.0, a=1200.0,
+                       ridge_xc=15 * 500.0, np1=1, seed=5)
+    tp = W.transport_problem(p, ncomp=3, fall_comp=1, mom_comps=(2,), met=bool(met),
+                             wind=25.0, noise=4.0)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    with open(fin, "wb") as f:
+        f.write(struct.pack("<9i", tp.nx, tp.ny, tp.nz, tp.ncomp, tp.iterg, met, 1, 1, 30))
+        f.write(struct.pack("<d", 1e-10))
+        for a in (tp.tmom, tp.tpot, tp.fall, tp.lu):
+            f.write(np.ascontiguousarray(a, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(tp.op_ps, dtype="<i4").tobytes())
+        f.write(np.ascontiguousarray(tp.alpha, dtype="<f8").tobytes())
+        f.write(np.ascontiguousarray(tp.b, dtype="<f8").tobytes())
+    r = subprocess.run([drv, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    with open(fout, "rb") as f:
+        maxiter, = struct.unpack("<i", f.read(4))
+        tol, = struct.unpack("<d", f.read(8))
+        its, = struct.unpack("<i", f.read(4))
+        x = np.frombuffer(f.read(8 * tp.ncomp * tp.m), dtype="<f8").reshape(tp.ncomp, tp.m)
+    xo, mo, to, so = pytransport.TransportOracle(tp).bicgstab(tp.b, 30, 1e-10)
+    err = np.linalg.norm(tp.interior(x) - tp.interior(xo)) / np.linalg.norm(tp.interior(xo))
+    print(f"Fortran BiCGStab{'Met' if met else 'Chem'}Sp: MaxIter {maxiter} (oracle {mo}), "
+          f"Tol {tol:.3e} (oracle {to:.3e}), err vs oracle {err:.2e}")
+    assert abs(abs(maxiter) - abs(mo)) <= 1 and (maxiter < 0) == (mo < 0)
+    assert err <= 1e-10
