@@ -15,14 +15,14 @@
 //
 // The Gauss-Seidel sweep is the reference's lexicographic forward sweep.  Cell (ix,iy,iz)
 // needs the new values of (ix-1), (iy-1), (iz-1) and the old ones of (ix+1), (iy+1),
-// (iz+1), so every cell of the hyperplane ix+iy+iz = h can be updated at once.  Rows are
-// pipelined: the wavefront of row iy runs one step behind the one of row iy-1.  Inside a
+// (iz+1), so every cell of the hyperplane ix+iy+iz = h can be updated at once.  A sweep is
+// t = b - U x_old (a parallel pass) followed by the forward substitution, pipelined over
+// rows: the wavefront of row iy runs one step behind the one of row iy-1.  Inside a
 // workgroup the new values of row iy-1 arrive through LDS (one barrier per step); between
-// workgroups through the vector itself, guarded by a progress flag that the last row of
-// each workgroup publishes every few steps (release) and the first row of the next polls
-// (acquire).  Workgroups take their row block from an atomic ticket, so a workgroup only
-// ever waits for one that started before it.  Every wait is bounded: past the limit the
-// kernel records an error and stops waiting, so every wave reaches the end.
+// workgroups through a boundary-row buffer whose entries carry their own readiness (a
+// sentinel until written).  Workgroups take their row block from an atomic ticket, so a
+// workgroup only ever waits for one that started before it.  Every wait is bounded: past
+// the limit the kernel records an error and stops waiting, so every wave reaches the end.
 //
 // Products and sums are not contracted (no FMA) in the operator, the sweep and the
 // vector updates, so they round as the reference's separate multiplies and adds.
@@ -177,170 +177,203 @@ __global__ void k_zero_int(int* p, int n) {
 }
 
 // ------------------------------------------------------------------ Gauss-Seidel sweep
+// A sweep is split as the reference's loops are (GaussSeidel_SpDiag :216-234): first
+// t = b - U x_old for every cell (k_gs_upart, fully parallel), then the forward
+// substitution (L+D) x = t (k_gs_lower, the pipelined part).  Term order of a row as in
+// the reference: b, -U (+1, +X, +XY, then JacFall's +XY), -L (-XY, -X, -1), / diag.
+constexpr unsigned long long kSentinel = 0xFFF7DEADBEEF0001ull;  // NaN payload: not yet
+
 struct GsArgs {
   Dims d;
-  const double* wmom;  // 7 x sks
+  const double* wmom;  // 7 x sks row coefficients (skewed)
   const double* wpot;  // 7 x sks
-  const double* wf;    // 2 x sks (JacFall rows) or null
-  const double* bs;    // nc x sks
-  double* xs;          // nc x sks, updated in place
+  const double* wf;    // 2 x sks JacFall rows
+  const double* bs;    // nc x sks right-hand side
+  const double* ts;    // nc x sks right-hand side of the forward substitution
+  double* tw;          // nc x sks: written by k_gs_upart
+  double* xs;          // nc x sks: old values in, new values out
+  double* bnd;         // nc x G x S x Z: last row of each block, for the next block
   int* counter;        // nc tickets
-  int* flags;          // nc x G progress of each block's last row
   int* err;            // set when a wait gives up
-  int G, sweep, publish;
+  int G;
 };
 
-template <int M>
-struct Pre {
-  double w[M][7];
-  double b[M];
-  double xo[M][3];  // old (ix+1), (iy+1), (iz+1)
-  double f[M][2];
-};
+// t = b - U x_old on every valid slot of every component (sweeps after the first).
+template <bool FALL>
+__global__ __launch_bounds__(256) void k_gs_upart(GsArgs a, Ops ops, int op_sel) {
+#pragma clang fp contract(off)
+  const Dims& d = a.d;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)ops.nc * d.sks;
+  if (e >= total) return;
+  const int c = (int)(e / d.sks);
+  const long k = e - (long)c * d.sks;
+  const int op = ops.op[c];
+  if ((op == ASAM_TR_FALL) != FALL || (op_sel >= 0 && op != op_sel)) return;
+  const int iz = (int)(k % d.Z);
+  const long t1 = k / d.Z;
+  const int s = (int)(t1 % d.S), iy = (int)(t1 / d.S), ix = s - iz;
+  if (ix < 0 || ix >= d.X) return;
+  const double* __restrict__ W = (op == ASAM_TR_MOM) ? a.wmom : a.wpot;
+  const double* __restrict__ x = a.xs + c * d.sks;
+  const long SZ = (long)d.S * d.Z;
+  double t = a.bs[c * d.sks + k];
+  if (ix + 1 < d.X) t = t - W[4 * d.sks + k] * x[k + d.Z];
+  if (iy + 1 < d.Y) t = t - W[5 * d.sks + k] * x[k + SZ];
+  if (iz + 1 < d.Z) {
+    t = t - W[6 * d.sks + k] * x[k + d.Z + 1];
+    if (FALL) t = t - (a.wf[d.sks + k] * ops.alpha[c]) * x[k + d.Z + 1];
+  }
+  a.tw[c * d.sks + k] = t;
+}
 
-// One forward sweep (L+D) x_new = b - U x_old; sweep 0 starts from x_old = 0 (the first
-// forward substitution of GaussSeidel_SpDiag :204-212).  Order of the terms of a row as
-// in the reference: b, -U (+1, +X, +XY, then JacFall's +XY), -L (-XY, -X, -1), / diag.
-//
-// Step T of the workgroup is step s = T - j of wave j.  The coefficients, b and the old
-// neighbours of step s are loaded D steps ahead into ring slot T mod D; the loop is
-// unrolled by D so every slot index is a constant (no register moves, so a step waits
-// only for the loads issued D steps before).  New values are stored with agent-scope
-// (write-through) stores, so the L2 holds no dirty lines of the vector and the release
-// that publishes a block's progress has little to write back.
-template <int M, int D>
-__global__ __launch_bounds__(512) void k_gs(GsArgs a, Ops ops) {
+__global__ void k_fill_u64(unsigned long long* p, long n, unsigned long long v) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) p[e] = v;
+}
+
+__device__ __forceinline__ bool is_sentinel(double v) {
+  return (unsigned long long)__double_as_longlong(v) == kSentinel;
+}
+
+// Forward substitution (L+D) x = t, pipelined over rows.  Step T of the workgroup is step
+// s = T - j of wave j (row iy = g*R + j).  The wave loads the t and L/D coefficients of C
+// steps at once (one memory round trip per C steps, not per step), then runs the C steps
+// from registers: new values of row iy-1 come through LDS (one barrier per step) or, for
+// the first row of a block, from the previous block's boundary row; the z-neighbour from
+// the lane below, the x-neighbour from the lane's own previous step.  The boundary rows
+// are written with agent-scope atomic stores into a buffer pre-filled with a NaN-payload
+// sentinel and read with agent-scope atomic loads; a value that is not the sentinel is
+// final, so no fence or flag orders them.  Blocks take their rows from an atomic ticket
+// and only wait for a block that started before them; every wait is bounded.
+template <int M, int C, bool FALL>
+__global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int op_sel) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];  // [2][R][Z]
   __shared__ int s_ticket;
   const int lane = threadIdx.x & 63, j = threadIdx.x >> 6, R = blockDim.x >> 6;
   const Dims& d = a.d;
   const int c = blockIdx.x / a.G;
+  const int op = ops.op[c];
+  // a launch serves the components of one kind (FALL or not); the others leave at once,
+  // whole workgroups together, before any barrier
+  if ((op == ASAM_TR_FALL) != FALL || (op_sel >= 0 && op != op_sel)) return;
   if (threadIdx.x == 0) s_ticket = atomicAdd(a.counter + c, 1);
   __syncthreads();
   const int g = s_ticket;
   const int iy = g * R + j;
   const bool row_ok = iy < d.Y;
-  const int op = ops.op[c];
   const double alpha = ops.alpha[c];
-  const bool fall = op == ASAM_TR_FALL;
   const double* __restrict__ W = (op == ASAM_TR_MOM) ? a.wmom : a.wpot;
   const long sks = d.sks;
-  const double* __restrict__ bs = a.bs + c * sks;
+  const double* __restrict__ ts = a.ts + c * sks;
   double* xs = a.xs + c * sks;
-  int* flag = a.flags + c * a.G;
-  const int S = d.S, Z = d.Z, X = d.X, Y = d.Y;
+  const int S = d.S, Z = d.Z, X = d.X;
   const long SZ = (long)S * Z;
   const long rowbase = (long)(row_ok ? iy : 0) * SZ;
-  const bool upd = a.sweep > 0;
-
-  // Loads are unconditional and branch-free: the slot index is clamped into the row and
-  // the skewed vector carries S*Z + Z + 1 entries of padding, so every address is valid.
-  // Values of slots or neighbours that do not exist are never used: the same conditions
-  // guard their use.  With no branches around the loads the wait counts stay exact and a
-  // step waits only for the loads issued D steps before it.
-  auto load = [&](int sl, Pre<M>& p) {
-    const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-#pragma unroll
-    for (int mm = 0; mm < M; ++mm) {
-      const int iz = lane + 64 * mm;
-      const long k = rowbase + (long)slc * Z + (iz < Z ? iz : Z - 1);
-#pragma unroll
-      for (int q = 0; q < 7; ++q) p.w[mm][q] = W[q * sks + k];
-      p.b[mm] = bs[k];
-      p.xo[mm][0] = xs[k + Z];
-      p.xo[mm][1] = xs[k + SZ];
-      p.xo[mm][2] = xs[k + Z + 1];
-      p.f[mm][0] = a.wf[k];
-      p.f[mm][1] = a.wf[sks + k];
-    }
-  };
-  Pre<M> pre[D];
-#pragma unroll
-  for (int ph = 0; ph < D; ++ph) load(ph - j, pre[ph]);  // slot of step T = ph
+  const bool publish = j == R - 1 && g + 1 < a.G && row_ok;
+  const bool consume = j == 0 && g > 0 && row_ok;
+  double* bnd_out = a.bnd + ((long)c * a.G + g) * SZ;
+  const double* bnd_in = a.bnd + ((long)c * a.G + (g > 0 ? g - 1 : 0)) * SZ;
 
   double prev[M];
 #pragma unroll
   for (int mm = 0; mm < M; ++mm) prev[mm] = 0.0;
-  int seen = 0;
   bool broken = false;
   const int steps = S + R - 1;
-  for (int T0 = 0; T0 < steps; T0 += D) {
+  for (int T0 = 0; T0 < steps; T0 += C) {
+    double tt[C][M], w0[C][M], w1[C][M], w2[C][M], w3[C][M], f0[C][M], by[C][M];
+    bool need[C][M];
+    // one batch of loads for C steps (addresses clamped into the row: always valid)
 #pragma unroll
-    for (int ph = 0; ph < D; ++ph) {
-      const int T = T0 + ph;
+    for (int q = 0; q < C; ++q) {
+      const int sl = T0 + q - j;
+      const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
+#pragma unroll
+      for (int mm = 0; mm < M; ++mm) {
+        const int iz = lane + 64 * mm;
+        const int izc = iz < Z ? iz : Z - 1;
+        const long k = rowbase + (long)slc * Z + izc;
+        tt[q][mm] = ts[k];
+        w0[q][mm] = W[k];
+        w1[q][mm] = W[sks + k];
+        w2[q][mm] = W[2 * sks + k];
+        w3[q][mm] = W[3 * sks + k];
+        f0[q][mm] = FALL ? a.wf[k] : 0.0;
+        const int ix = sl - iz;
+        need[q][mm] = consume && sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
+        by[q][mm] = consume ? __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                            : 0.0;
+      }
+    }
+    // the first row of a block waits until the previous block's last row has all C steps
+    if (consume && !broken) {
+      int n = 0;
+      while (true) {
+        bool miss = false;
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+#pragma unroll
+          for (int mm = 0; mm < M; ++mm) miss |= need[q][mm] && is_sentinel(by[q][mm]);
+        if (!__any(miss)) break;
+        if (++n > kSpinLimit) {
+          __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          broken = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+          const int sl = T0 + q - j;
+          const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
+#pragma unroll
+          for (int mm = 0; mm < M; ++mm) {
+            const int iz = lane + 64 * mm;
+            const int izc = iz < Z ? iz : Z - 1;
+            if (need[q][mm] && is_sentinel(by[q][mm]))
+              by[q][mm] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int T = T0 + q;
       if (T >= steps) break;  // uniform across the workgroup
       const int s = T - j;
       const bool active = row_ok && s >= 0 && s < S;
       double up[M];
 #pragma unroll
       for (int mm = 0; mm < M; ++mm) up[mm] = __shfl(prev[mm], (lane + 63) & 63);
-      // first row of a block after the first: row iy-1 of the previous block must have
-      // finished step s
-      if (j == 0 && g > 0 && active && !broken && seen < s + 1) {
-        int v = 0;
-        if (lane == 0) {
-          int n = 0;
-          while ((v = __hip_atomic_load(flag + g - 1, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT)) < s + 1) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++n > kSpinLimit) {
-              __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              v = -1;
-              break;
-            }
-          }
-        }
-        v = __shfl(v, 0);
-        if (v < 0) {
-          broken = true;
-        } else {
-          seen = v;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
       const int pb = ((T - 1) & 1) * R * Z, cb = (T & 1) * R * Z;
-      const Pre<M>& p = pre[ph];
-      double out[M];
 #pragma unroll
       for (int mm = 0; mm < M; ++mm) {
-        out[mm] = 0.0;
         const int iz = lane + 64 * mm, ix = s - iz;
         const bool valid = active && iz < Z && ix >= 0 && ix < X;
-        // row iy-1: from the wave above through LDS, or from the previous block
-        double xy_l = 0.0, xy_g = 0.0;
+        double xy_l = 0.0;
         if (j > 0 && iz < Z) xy_l = lds[pb + (j - 1) * Z + iz];
-        if (j == 0 && valid && iy > 0) xy_g = xs[rowbase + (long)s * Z + iz - SZ];
+        double out = 0.0;
         if (valid) {
-          const long k = rowbase + (long)s * Z + iz;
-          double t = p.b[mm];
-          if (upd) {
-            if (ix + 1 < X) t = t - p.w[mm][4] * p.xo[mm][0];
-            if (iy + 1 < Y) t = t - p.w[mm][5] * p.xo[mm][1];
-            if (iz + 1 < Z) {
-              t = t - p.w[mm][6] * p.xo[mm][2];
-              if (fall) t = t - (p.f[mm][1] * alpha) * p.xo[mm][2];
-            }
-          }
+          double t = tt[q][mm];
           if (iz > 0) {
             const double xz = lane > 0 ? up[mm] : up[mm > 0 ? mm - 1 : 0];
-            t = t - p.w[mm][0] * xz;
+            t = t - w0[q][mm] * xz;
           }
-          if (iy > 0) t = t - p.w[mm][1] * (j > 0 ? xy_l : xy_g);
-          if (ix > 0) t = t - p.w[mm][2] * prev[mm];
-          double dg = p.w[mm][3];
-          if (fall) dg = dg + p.f[mm][0] * alpha;
-          out[mm] = t / dg;
-          __hip_atomic_store(xs + k, out[mm], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (iy > 0) t = t - w1[q][mm] * (j > 0 ? xy_l : by[q][mm]);
+          if (ix > 0) t = t - w2[q][mm] * prev[mm];
+          double dg = w3[q][mm];
+          if (FALL) dg = dg + f0[q][mm] * alpha;
+          out = t / dg;
+          const long k = rowbase + (long)s * Z + iz;
+          xs[k] = out;
+          if (publish)
+            __hip_atomic_store(bnd_out + (long)s * Z + iz, out, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (iz < Z) lds[cb + j * Z + iz] = out[mm];
-        prev[mm] = out[mm];
-      }
-      load(s + D, pre[ph]);
-      if (j == R - 1 && g + 1 < a.G && active && ((s + 1) % a.publish == 0 || s == S - 1)) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        if (lane == 0)
-          __hip_atomic_store(flag + g, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (iz < Z) lds[cb + j * Z + iz] = out;
+        prev[mm] = out;
       }
       __syncthreads();
     }
@@ -408,17 +441,17 @@ struct asam_tr_ctx {
   Dims d{};
   int nc = 0, iterg = 5, M = 1;
   Ops ops_mv{}, ops_ps{};  // per-component operator of MatVec and of PSolve
-  asam_tr_tuning tune{8, 2, 4, 0};
+  asam_tr_tuning tune{8, 8, 0, 0};
   bool have_mom = false, have_pot = false, have_fall = false, have_lu = false;
   std::string err;
   std::vector<void*> allocs;
   double *tmom = nullptr, *tpot = nullptr, *fall = nullptr, *lu = nullptr;
   double *wmom = nullptr, *wpot = nullptr, *wf = nullptr;
-  double *bs = nullptr, *xs = nullptr;
+  double *bs = nullptr, *xs = nullptr, *ts = nullptr, *bnd = nullptr;
   double *vx = nullptr, *vb = nullptr, *vr = nullptr, *vrt = nullptr, *vrh = nullptr;
   double *vp = nullptr, *vph = nullptr, *vt = nullptr, *vv = nullptr, *vh = nullptr;
   double *partial = nullptr, *dots = nullptr;
-  int *sync = nullptr;  // counters, flags, error word
+  int *sync = nullptr;  // tickets, error word
   int sync_n = 0;
   double* host_dots = nullptr;  // pinned
 };
@@ -519,12 +552,14 @@ extern "C" int asam_tr_create(asam_tr_ctx** out, const asam_tr_grid* g) {
   rc |= tralloc(c, &c->wf, 2 * (size_t)d.sks);
   rc |= tralloc(c, &c->bs, (size_t)c->nc * d.sks);
   rc |= tralloc(c, &c->xs, (size_t)c->nc * d.sks + (size_t)d.S * d.Z + d.Z + 1);
+  rc |= tralloc(c, &c->ts, (size_t)c->nc * d.sks);
+  rc |= tralloc(c, &c->bnd, (size_t)c->nc * d.sks);
   double** vecs[] = {&c->vx, &c->vb, &c->vr, &c->vrt, &c->vrh, &c->vp, &c->vph, &c->vt, &c->vv,
                      &c->vh};
   for (double** v : vecs) rc |= tralloc(c, v, V);
   rc |= tralloc(c, &c->partial, 2 * (size_t)kDotBlocks);
   rc |= tralloc(c, &c->dots, 2);
-  c->sync_n = c->nc * (1 + (int)Y) + 1;  // counters, flags (G <= Y), error word
+  c->sync_n = c->nc + 1;  // tickets, error word
   rc |= tralloc(c, &c->sync, (size_t)c->sync_n);
   if (rc) return bail(ASAM_E_NOMEM);
   if (hipHostMalloc((void**)&c->host_dots, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess)
@@ -663,35 +698,56 @@ static int launch_matvec(asam_tr_ctx* c, const double* x, double* y, int init) {
   return 0;
 }
 
-template <int M, int D>
-static void gs_launch(asam_tr_ctx* c, const GsArgs& a, int R) {
+template <int M, int C, bool FALL>
+static void lower_launch(asam_tr_ctx* c, const GsArgs& a, int R) {
   const size_t lds = 2 * (size_t)R * c->d.Z * sizeof(double);
-  k_gs<M, D><<<(unsigned)(c->nc * a.G), 64 * R, lds, c->stream>>>(a, c->ops_ps);
+  k_gs_lower<M, C, FALL><<<(unsigned)(c->nc * a.G), 64 * R, lds, c->stream>>>(a, c->ops_ps, -1);
 }
 
-// One forward sweep over every component (counters and flags cleared first).
-static int launch_sweep(asam_tr_ctx* c, int sweep) {
-  const int R = rows_per_wg(c), G = blocks_y(c);
-  const int nsync = c->nc * (1 + G);
-  k_zero_int<<<blocks_for(nsync, 256), 256, 0, c->stream>>>(c->sync, nsync);
-  GsArgs a;
-  a.d = c->d;
-  a.wmom = c->wmom, a.wpot = c->wpot, a.wf = c->wf;
-  a.bs = c->bs, a.xs = c->xs;
-  a.counter = c->sync;
-  a.flags = c->sync + c->nc;
-  a.err = c->sync + c->sync_n - 1;
-  a.G = G, a.sweep = sweep;
-  a.publish = c->tune.gs_publish > 0 ? c->tune.gs_publish : 1;
-  const int D = c->tune.gs_prefetch;
+template <bool FALL>
+static void lower_kind(asam_tr_ctx* c, const GsArgs& a, int R) {
+  const int C = c->tune.gs_chunk;
   if (c->M == 1) {
-    if (D >= 4) gs_launch<1, 4>(c, a, R);
-    else if (D >= 2) gs_launch<1, 2>(c, a, R);
-    else gs_launch<1, 1>(c, a, R);
-  } else {  // two z-slots per lane: at most 2 steps ahead (register budget)
-    if (D >= 2) gs_launch<2, 2>(c, a, R);
-    else gs_launch<2, 1>(c, a, R);
+    if (C >= 8) lower_launch<1, 8, FALL>(c, a, R);
+    else lower_launch<1, 4, FALL>(c, a, R);
+  } else {  // two z-slots per lane: chunks of 4 (register budget)
+    lower_launch<2, 4, FALL>(c, a, R);
   }
+}
+
+// One Gauss-Seidel sweep over every component: t = b - U x_old (not for the first
+// sweep, which starts from x = 0), then the pipelined forward substitution.  Tickets are
+// cleared and the boundary rows reset to the sentinel first.
+static int launch_sweep(asam_tr_ctx* c, int sweep) {
+  const Dims& d = c->d;
+  const int R = rows_per_wg(c), G = blocks_y(c);
+  const bool any_fall = uses(c->ops_ps, c->nc, ASAM_TR_FALL) &&
+                        [&] {
+                          for (int q = 0; q < c->nc; ++q)
+                            if (c->ops_ps.op[q] == ASAM_TR_FALL) return true;
+                          return false;
+                        }();
+  bool any_plain = false;
+  for (int q = 0; q < c->nc; ++q) any_plain |= c->ops_ps.op[q] != ASAM_TR_FALL;
+  GsArgs a;
+  a.d = d;
+  a.wmom = c->wmom, a.wpot = c->wpot, a.wf = c->wf;
+  a.bs = c->bs, a.tw = c->ts, a.xs = c->xs, a.bnd = c->bnd;
+  a.ts = sweep > 0 ? c->ts : c->bs;
+  a.counter = c->sync;
+  a.err = c->sync + c->sync_n - 1;
+  a.G = G;
+  if (sweep > 0) {
+    const long V = (long)c->nc * d.sks;
+    if (any_plain) k_gs_upart<false><<<blocks_for(V, 256), 256, 0, c->stream>>>(a, c->ops_ps, -1);
+    if (any_fall) k_gs_upart<true><<<blocks_for(V, 256), 256, 0, c->stream>>>(a, c->ops_ps, -1);
+  }
+  k_zero_int<<<blocks_for(c->nc, 256), 256, 0, c->stream>>>(c->sync, c->nc);
+  const long nb = (long)c->nc * G * d.S * d.Z;
+  k_fill_u64<<<blocks_for(nb, 256), 256, 0, c->stream>>>((unsigned long long*)c->bnd, nb,
+                                                          kSentinel);
+  if (any_plain) lower_kind<false>(c, a, R);
+  if (any_fall) lower_kind<true>(c, a, R);
   TRCHK(c, hipGetLastError());
   return 0;
 }
@@ -914,9 +970,7 @@ extern "C" int asam_tr_get_tuning(const asam_tr_ctx* c, asam_tr_tuning* t) {
 extern "C" int asam_tr_set_tuning(asam_tr_ctx* c, const asam_tr_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
   if (t->gs_rows < 1 || t->gs_rows > 8) return trfail(c, ASAM_E_INVALID, "gs_rows");
-  if (t->gs_prefetch != 1 && t->gs_prefetch != 2 && t->gs_prefetch != 4)
-    return trfail(c, ASAM_E_INVALID, "gs_prefetch");
-  if (t->gs_publish < 1) return trfail(c, ASAM_E_INVALID, "gs_publish");
+  if (t->gs_chunk != 4 && t->gs_chunk != 8) return trfail(c, ASAM_E_INVALID, "gs_chunk");
   if (2 * (size_t)t->gs_rows * c->d.Z * sizeof(double) > 60 * 1024)
     return trfail(c, ASAM_E_INVALID, "gs_rows: LDS");
   c->tune = *t;

@@ -38,8 +38,7 @@ class TrInfo(C.Structure):
 
 
 class TrTuning(C.Structure):
-    _fields_ = [("gs_rows", C.c_int), ("gs_prefetch", C.c_int), ("gs_publish", C.c_int),
-                ("reserved", C.c_int)]
+    _fields_ = [("gs_rows", C.c_int), ("gs_chunk", C.c_int), ("reserved", C.c_int * 2)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

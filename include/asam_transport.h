@@ -71,10 +71,10 @@ typedef struct {
 
 /* Launch shapes of the Gauss-Seidel sweep (results do not depend on them). */
 typedef struct {
-  int gs_rows;     /* y-rows per workgroup (one wavefront each): 1..8 */
-  int gs_prefetch; /* steps of coefficients loaded ahead: 1, 2 or 4 (2 when nz > 62) */
-  int gs_publish;  /* steps between progress flags of a workgroup's last row: >= 1 */
-  int reserved;
+  int gs_rows;  /* y-rows per workgroup (one wavefront each): 1..8 */
+  int gs_chunk; /* steps whose coefficients a wavefront loads in one batch: 4 or 8
+                   (4 when nz > 62) */
+  int reserved[2];
 } asam_tr_tuning;
 
 /* Creates the context and its device workspace (no per-call allocation). */

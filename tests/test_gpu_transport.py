@@ -3,7 +3,7 @@
     equal the oracle bit for bit (both round every product and sum separately, in the
     reference's order) on ragged grids, with one or two z-slots per lane,
     JacTMom / JacFall components, and every launch shape of the pipelined sweep (one row
-    per workgroup exercises the cross-workgroup progress flags on every row);
+    per workgroup hands every row over between workgroups);
   - BiCGStab (Chem and Met forms) matches the oracle's solution within 1e-10 relative,
     the iteration count within one, and the MaxIter/Tol return conventions;
   - device entry points equal the host ones; couplings across the linear-index wrap are
@@ -60,9 +60,8 @@ def test_matvec_equals_oracle(shape, trm, orc):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("tune", [dict(gs_rows=8, gs_prefetch=2, gs_publish=4),
-                                  dict(gs_rows=1, gs_prefetch=1, gs_publish=1),
-                                  dict(gs_rows=3, gs_prefetch=4, gs_publish=7)])
+@pytest.mark.parametrize("tune", [dict(gs_rows=8, gs_chunk=8), dict(gs_rows=1, gs_chunk=4),
+                                  dict(gs_rows=3, gs_chunk=8), dict(gs_rows=5, gs_chunk=4)])
 def test_psolve_equals_oracle(shape, tune, trm, orc):
     tp = problem(*shape, ncomp=3, fall_comp=1, mom_comps=(2,))
     s = trm.TransportSolver.from_problem(tp)
