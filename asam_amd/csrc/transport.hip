@@ -198,33 +198,41 @@ struct GsArgs {
   int G;
 };
 
-// t = b - U x_old on every valid slot of every component (sweeps after the first).
-template <bool FALL>
-__global__ __launch_bounds__(256) void k_gs_upart(GsArgs a, Ops ops, int op_sel) {
+// t = b - U x_old on every valid slot (sweeps after the first), all components of a cell
+// in one thread so the U coefficients are read once per cell.
+template <int NC>
+__global__ __launch_bounds__(256) void k_gs_upart(GsArgs a, Ops ops, int use_mom, int use_pot) {
 #pragma clang fp contract(off)
   const Dims& d = a.d;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)ops.nc * d.sks;
-  if (e >= total) return;
-  const int c = (int)(e / d.sks);
-  const long k = e - (long)c * d.sks;
-  const int op = ops.op[c];
-  if ((op == ASAM_TR_FALL) != FALL || (op_sel >= 0 && op != op_sel)) return;
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.sks) return;
   const int iz = (int)(k % d.Z);
   const long t1 = k / d.Z;
   const int s = (int)(t1 % d.S), iy = (int)(t1 / d.S), ix = s - iz;
   if (ix < 0 || ix >= d.X) return;
-  const double* __restrict__ W = (op == ASAM_TR_MOM) ? a.wmom : a.wpot;
-  const double* __restrict__ x = a.xs + c * d.sks;
   const long SZ = (long)d.S * d.Z;
-  double t = a.bs[c * d.sks + k];
-  if (ix + 1 < d.X) t = t - W[4 * d.sks + k] * x[k + d.Z];
-  if (iy + 1 < d.Y) t = t - W[5 * d.sks + k] * x[k + SZ];
-  if (iz + 1 < d.Z) {
-    t = t - W[6 * d.sks + k] * x[k + d.Z + 1];
-    if (FALL) t = t - (a.wf[d.sks + k] * ops.alpha[c]) * x[k + d.Z + 1];
+  const bool ex = ix + 1 < d.X, ey = iy + 1 < d.Y, ez = iz + 1 < d.Z;
+  double up[3] = {0.0, 0.0, 0.0}, um[3] = {0.0, 0.0, 0.0};
+  if (use_pot) {
+    up[0] = a.wpot[4 * d.sks + k], up[1] = a.wpot[5 * d.sks + k], up[2] = a.wpot[6 * d.sks + k];
   }
-  a.tw[c * d.sks + k] = t;
+  if (use_mom) {
+    um[0] = a.wmom[4 * d.sks + k], um[1] = a.wmom[5 * d.sks + k], um[2] = a.wmom[6 * d.sks + k];
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int op = ops.op[c];
+    const double* w = op == ASAM_TR_MOM ? um : up;
+    const double* __restrict__ x = a.xs + c * d.sks;
+    double t = a.bs[c * d.sks + k];
+    if (ex) t = t - w[0] * x[k + d.Z];
+    if (ey) t = t - w[1] * x[k + SZ];
+    if (ez) {
+      t = t - w[2] * x[k + d.Z + 1];
+      if (op == ASAM_TR_FALL) t = t - (a.wf[d.sks + k] * ops.alpha[c]) * x[k + d.Z + 1];
+    }
+    a.tw[c * d.sks + k] = t;
+  }
 }
 
 __global__ void k_fill_u64(unsigned long long* p, long n, unsigned long long v) {
@@ -236,28 +244,36 @@ __device__ __forceinline__ bool is_sentinel(double v) {
   return (unsigned long long)__double_as_longlong(v) == kSentinel;
 }
 
-// Forward substitution (L+D) x = t, pipelined over rows.  Step T of the workgroup is step
-// s = T - j of wave j (row iy = g*R + j).  The wave loads the t and L/D coefficients of C
-// steps at once (one memory round trip per C steps, not per step), then runs the C steps
-// from registers: new values of row iy-1 come through LDS (one barrier per step) or, for
-// the first row of a block, from the previous block's boundary row; the z-neighbour from
-// the lane below, the x-neighbour from the lane's own previous step.  The boundary rows
-// are written with agent-scope atomic stores into a buffer pre-filled with a NaN-payload
-// sentinel and read with agent-scope atomic loads; a value that is not the sentinel is
-// final, so no fence or flag orders them.  Blocks take their rows from an atomic ticket
-// and only wait for a block that started before them; every wait is bounded.
-template <int M, int C, bool FALL>
-__global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int op_sel) {
+// Forward substitution (L+D) x = t, pipelined over rows.  A row is Q = ceil(Z/64)
+// wavefronts, lane iz = 64*qz + lane; step T of the workgroup is step s = T - j of row j
+// (row iy = g*R + j).  Each wavefront loads the t and L/D coefficients of C steps at once,
+// double-buffered: the loads of the next C steps are in flight while the current C steps
+// run from registers.  New values of row iy-1 come through LDS (one barrier per step) or,
+// for the first row of a block, from the previous block's boundary row; the z-neighbour
+// from the lane below (through LDS across the two wavefronts of a row), the x-neighbour
+// from the lane's own previous step.  Boundary rows are written with agent-scope atomic
+// stores into a buffer pre-filled with a NaN-payload sentinel and read with agent-scope
+// atomic loads; a value that is not the sentinel is final, so no fence or flag orders
+// them.  Blocks take their rows from an atomic ticket and only wait for a block that
+// started before them; every wait is bounded.
+template <int C>
+struct Chunk {
+  double t[C], w0[C], w1[C], w2[C], w3[C], f0[C], by[C];
+};
+
+template <int C, bool FALL>
+__global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
 #pragma clang fp contract(off)
   extern __shared__ double lds[];  // [2][R][Z]
   __shared__ int s_ticket;
-  const int lane = threadIdx.x & 63, j = threadIdx.x >> 6, R = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int j = wv / Q, qz = wv - j * Q, R = (blockDim.x >> 6) / Q;
   const Dims& d = a.d;
   const int c = blockIdx.x / a.G;
   const int op = ops.op[c];
-  // a launch serves the components of one kind (FALL or not); the others leave at once,
-  // whole workgroups together, before any barrier
-  if ((op == ASAM_TR_FALL) != FALL || (op_sel >= 0 && op != op_sel)) return;
+  // a launch serves the components of one kind (with JacFall or not); the others leave
+  // at once, whole workgroups together, before any barrier
+  if ((op == ASAM_TR_FALL) != FALL) return;
   if (threadIdx.x == 0) s_ticket = atomicAdd(a.counter + c, 1);
   __syncthreads();
   const int g = s_ticket;
@@ -271,51 +287,47 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int op_sel)
   const int S = d.S, Z = d.Z, X = d.X;
   const long SZ = (long)S * Z;
   const long rowbase = (long)(row_ok ? iy : 0) * SZ;
+  const int iz = 64 * qz + lane;
+  const int izc = iz < Z ? iz : Z - 1;
   const bool publish = j == R - 1 && g + 1 < a.G && row_ok;
   const bool consume = j == 0 && g > 0 && row_ok;
   double* bnd_out = a.bnd + ((long)c * a.G + g) * SZ;
   const double* bnd_in = a.bnd + ((long)c * a.G + (g > 0 ? g - 1 : 0)) * SZ;
-
-  double prev[M];
-#pragma unroll
-  for (int mm = 0; mm < M; ++mm) prev[mm] = 0.0;
-  bool broken = false;
   const int steps = S + R - 1;
-  for (int T0 = 0; T0 < steps; T0 += C) {
-    double tt[C][M], w0[C][M], w1[C][M], w2[C][M], w3[C][M], f0[C][M], by[C][M];
-    bool need[C][M];
-    // one batch of loads for C steps (addresses clamped into the row: always valid)
+
+  auto load = [&](int T0, Chunk<C>& ch) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int sl = T0 + q - j;
       const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-#pragma unroll
-      for (int mm = 0; mm < M; ++mm) {
-        const int iz = lane + 64 * mm;
-        const int izc = iz < Z ? iz : Z - 1;
-        const long k = rowbase + (long)slc * Z + izc;
-        tt[q][mm] = ts[k];
-        w0[q][mm] = W[k];
-        w1[q][mm] = W[sks + k];
-        w2[q][mm] = W[2 * sks + k];
-        w3[q][mm] = W[3 * sks + k];
-        f0[q][mm] = FALL ? a.wf[k] : 0.0;
-        const int ix = sl - iz;
-        need[q][mm] = consume && sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
-        by[q][mm] = consume ? __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT)
-                            : 0.0;
-      }
+      const long k = rowbase + (long)slc * Z + izc;
+      ch.t[q] = ts[k];
+      ch.w0[q] = W[k];
+      ch.w1[q] = W[sks + k];
+      ch.w2[q] = W[2 * sks + k];
+      ch.w3[q] = W[3 * sks + k];
+      ch.f0[q] = FALL ? a.wf[k] : 0.0;
+      ch.by[q] = consume ? __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT)
+                         : 0.0;
     }
+  };
+  double prev = 0.0;
+  bool broken = false;
+  // runs the C steps of ch; the loads of the chunk after (nT0 into nx) are issued once
+  // its first step is done, so that step's waits do not include them
+  auto run = [&](int T0, Chunk<C>& ch, int nT0, Chunk<C>& nx) {
     // the first row of a block waits until the previous block's last row has all C steps
     if (consume && !broken) {
       int n = 0;
       while (true) {
         bool miss = false;
 #pragma unroll
-        for (int q = 0; q < C; ++q)
-#pragma unroll
-          for (int mm = 0; mm < M; ++mm) miss |= need[q][mm] && is_sentinel(by[q][mm]);
+        for (int q = 0; q < C; ++q) {
+          const int sl = T0 + q - j, ix = sl - iz;
+          const bool need = sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
+          miss |= need && is_sentinel(ch.by[q]);
+        }
         if (!__any(miss)) break;
         if (++n > kSpinLimit) {
           __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -327,56 +339,53 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int op_sel)
         for (int q = 0; q < C; ++q) {
           const int sl = T0 + q - j;
           const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-#pragma unroll
-          for (int mm = 0; mm < M; ++mm) {
-            const int iz = lane + 64 * mm;
-            const int izc = iz < Z ? iz : Z - 1;
-            if (need[q][mm] && is_sentinel(by[q][mm]))
-              by[q][mm] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-          }
+          if (is_sentinel(ch.by[q]))
+            ch.by[q] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
         }
       }
+      // settle the boundary values inside this branch (an empty register constraint, no
+      // instruction), so no load of them is still counted as pending where the
+      // wavefronts that do not consume join again
+#pragma unroll
+      for (int q = 0; q < C; ++q) asm volatile("" : "+v"(ch.by[q]));
     }
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int T = T0 + q;
       if (T >= steps) break;  // uniform across the workgroup
-      const int s = T - j;
-      const bool active = row_ok && s >= 0 && s < S;
-      double up[M];
-#pragma unroll
-      for (int mm = 0; mm < M; ++mm) up[mm] = __shfl(prev[mm], (lane + 63) & 63);
+      const int s = T - j, ix = s - iz;
+      const bool valid = row_ok && s >= 0 && s < S && iz < Z && ix >= 0 && ix < X;
+      const double below = __shfl(prev, (lane + 63) & 63);
       const int pb = ((T - 1) & 1) * R * Z, cb = (T & 1) * R * Z;
-#pragma unroll
-      for (int mm = 0; mm < M; ++mm) {
-        const int iz = lane + 64 * mm, ix = s - iz;
-        const bool valid = active && iz < Z && ix >= 0 && ix < X;
-        double xy_l = 0.0;
-        if (j > 0 && iz < Z) xy_l = lds[pb + (j - 1) * Z + iz];
-        double out = 0.0;
-        if (valid) {
-          double t = tt[q][mm];
-          if (iz > 0) {
-            const double xz = lane > 0 ? up[mm] : up[mm > 0 ? mm - 1 : 0];
-            t = t - w0[q][mm] * xz;
-          }
-          if (iy > 0) t = t - w1[q][mm] * (j > 0 ? xy_l : by[q][mm]);
-          if (ix > 0) t = t - w2[q][mm] * prev[mm];
-          double dg = w3[q][mm];
-          if (FALL) dg = dg + f0[q][mm] * alpha;
-          out = t / dg;
-          const long k = rowbase + (long)s * Z + iz;
-          xs[k] = out;
-          if (publish)
-            __hip_atomic_store(bnd_out + (long)s * Z + iz, out, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (iz < Z) lds[cb + j * Z + iz] = out;
-        prev[mm] = out;
+      double xy_l = 0.0, xz_l = 0.0;
+      if (j > 0 && iz < Z) xy_l = lds[pb + (j - 1) * Z + iz];
+      if (lane == 0 && qz > 0) xz_l = lds[pb + j * Z + iz - 1];
+      double out = 0.0;
+      if (valid) {
+        double t = ch.t[q];
+        if (iz > 0) t = t - ch.w0[q] * (lane > 0 ? below : xz_l);
+        if (iy > 0) t = t - ch.w1[q] * (j > 0 ? xy_l : ch.by[q]);
+        if (ix > 0) t = t - ch.w2[q] * prev;
+        double dg = ch.w3[q];
+        if (FALL) dg = dg + ch.f0[q] * alpha;
+        out = t / dg;
+        xs[rowbase + (long)s * Z + iz] = out;
+        if (publish)
+          __hip_atomic_store(bnd_out + (long)s * Z + iz, out, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
       }
+      if (iz < Z) lds[cb + j * Z + iz] = out;
+      prev = out;
       __syncthreads();
+      if (q == 0) load(nT0, nx);
     }
+  };
+  Chunk<C> A, B;
+  load(0, A);
+  for (int T0 = 0; T0 < steps; T0 += 2 * C) {
+    run(T0, A, T0 + C, B);
+    if (T0 + C < steps) run(T0 + C, B, T0 + 2 * C, A);
   }
 }
 
@@ -441,7 +450,7 @@ struct asam_tr_ctx {
   Dims d{};
   int nc = 0, iterg = 5, M = 1;
   Ops ops_mv{}, ops_ps{};  // per-component operator of MatVec and of PSolve
-  asam_tr_tuning tune{8, 8, 0, 0};
+  asam_tr_tuning tune{8, 2, 0, 0};
   bool have_mom = false, have_pot = false, have_fall = false, have_lu = false;
   std::string err;
   std::vector<void*> allocs;
@@ -485,10 +494,11 @@ static int tralloc(asam_tr_ctx* c, T** p, size_t count) {
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Rows per workgroup: at most 8 wavefronts (M per row, M = 2 when Z > 64).
 static int rows_per_wg(const asam_tr_ctx* c) {
   int R = c->tune.gs_rows;
   if (R < 1) R = 1;
-  if (R > 8) R = 8;
+  if (R > 8 / c->M) R = 8 / c->M;
   return R;
 }
 static int blocks_y(const asam_tr_ctx* c) {
@@ -698,21 +708,39 @@ static int launch_matvec(asam_tr_ctx* c, const double* x, double* y, int init) {
   return 0;
 }
 
-template <int M, int C, bool FALL>
-static void lower_launch(asam_tr_ctx* c, const GsArgs& a, int R) {
+template <int C, bool FALL>
+static void lower_launch(asam_tr_ctx* c, const GsArgs& a, int R, int Q) {
   const size_t lds = 2 * (size_t)R * c->d.Z * sizeof(double);
-  k_gs_lower<M, C, FALL><<<(unsigned)(c->nc * a.G), 64 * R, lds, c->stream>>>(a, c->ops_ps, -1);
+  k_gs_lower<C, FALL><<<(unsigned)(c->nc * a.G), 64 * R * Q, lds, c->stream>>>(a, c->ops_ps, Q);
 }
 
 template <bool FALL>
 static void lower_kind(asam_tr_ctx* c, const GsArgs& a, int R) {
-  const int C = c->tune.gs_chunk;
-  if (c->M == 1) {
-    if (C >= 8) lower_launch<1, 8, FALL>(c, a, R);
-    else lower_launch<1, 4, FALL>(c, a, R);
-  } else {  // two z-slots per lane: chunks of 4 (register budget)
-    lower_launch<2, 4, FALL>(c, a, R);
+  if (c->tune.gs_chunk >= 4) lower_launch<4, FALL>(c, a, R, c->M);
+  else lower_launch<2, FALL>(c, a, R, c->M);
+}
+
+template <int NC>
+static void upart_nc(asam_tr_ctx* c, const GsArgs& a, int use_mom, int use_pot) {
+  k_gs_upart<NC><<<blocks_for(c->d.sks, 256), 256, 0, c->stream>>>(a, c->ops_ps, use_mom,
+                                                                     use_pot);
+}
+
+static int launch_upart(asam_tr_ctx* c, const GsArgs& a) {
+  int mom = 0, pot = 0;
+  for (int q = 0; q < c->nc; ++q) (c->ops_ps.op[q] == ASAM_TR_MOM ? mom : pot) = 1;
+  switch (c->nc) {
+#define NCCASE(n) \
+  case n:         \
+    upart_nc<n>(c, a, mom, pot); \
+    break;
+    NCCASE(1) NCCASE(2) NCCASE(3) NCCASE(4) NCCASE(5) NCCASE(6) NCCASE(7) NCCASE(8)
+    NCCASE(9) NCCASE(10) NCCASE(11) NCCASE(12) NCCASE(13) NCCASE(14) NCCASE(15) NCCASE(16)
+#undef NCCASE
+    default:
+      return trfail(c, ASAM_E_INVALID, "ncomp");
   }
+  return 0;
 }
 
 // One Gauss-Seidel sweep over every component: t = b - U x_old (not for the first
@@ -721,14 +749,9 @@ static void lower_kind(asam_tr_ctx* c, const GsArgs& a, int R) {
 static int launch_sweep(asam_tr_ctx* c, int sweep) {
   const Dims& d = c->d;
   const int R = rows_per_wg(c), G = blocks_y(c);
-  const bool any_fall = uses(c->ops_ps, c->nc, ASAM_TR_FALL) &&
-                        [&] {
-                          for (int q = 0; q < c->nc; ++q)
-                            if (c->ops_ps.op[q] == ASAM_TR_FALL) return true;
-                          return false;
-                        }();
-  bool any_plain = false;
-  for (int q = 0; q < c->nc; ++q) any_plain |= c->ops_ps.op[q] != ASAM_TR_FALL;
+  bool any_fall = false, any_plain = false;
+  for (int q = 0; q < c->nc; ++q)
+    (c->ops_ps.op[q] == ASAM_TR_FALL ? any_fall : any_plain) = true;
   GsArgs a;
   a.d = d;
   a.wmom = c->wmom, a.wpot = c->wpot, a.wf = c->wf;
@@ -737,11 +760,8 @@ static int launch_sweep(asam_tr_ctx* c, int sweep) {
   a.counter = c->sync;
   a.err = c->sync + c->sync_n - 1;
   a.G = G;
-  if (sweep > 0) {
-    const long V = (long)c->nc * d.sks;
-    if (any_plain) k_gs_upart<false><<<blocks_for(V, 256), 256, 0, c->stream>>>(a, c->ops_ps, -1);
-    if (any_fall) k_gs_upart<true><<<blocks_for(V, 256), 256, 0, c->stream>>>(a, c->ops_ps, -1);
-  }
+  if (sweep > 0)
+    if (int rc = launch_upart(c, a)) return rc;
   k_zero_int<<<blocks_for(c->nc, 256), 256, 0, c->stream>>>(c->sync, c->nc);
   const long nb = (long)c->nc * G * d.S * d.Z;
   k_fill_u64<<<blocks_for(nb, 256), 256, 0, c->stream>>>((unsigned long long*)c->bnd, nb,
@@ -970,7 +990,7 @@ extern "C" int asam_tr_get_tuning(const asam_tr_ctx* c, asam_tr_tuning* t) {
 extern "C" int asam_tr_set_tuning(asam_tr_ctx* c, const asam_tr_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
   if (t->gs_rows < 1 || t->gs_rows > 8) return trfail(c, ASAM_E_INVALID, "gs_rows");
-  if (t->gs_chunk != 4 && t->gs_chunk != 8) return trfail(c, ASAM_E_INVALID, "gs_chunk");
+  if (t->gs_chunk != 2 && t->gs_chunk != 4) return trfail(c, ASAM_E_INVALID, "gs_chunk");
   if (2 * (size_t)t->gs_rows * c->d.Z * sizeof(double) > 60 * 1024)
     return trfail(c, ASAM_E_INVALID, "gs_rows: LDS");
   c->tune = *t;

@@ -71,9 +71,9 @@ typedef struct {
 
 /* Launch shapes of the Gauss-Seidel sweep (results do not depend on them). */
 typedef struct {
-  int gs_rows;  /* y-rows per workgroup (one wavefront each): 1..8 */
-  int gs_chunk; /* steps whose coefficients a wavefront loads in one batch: 4 or 8
-                   (4 when nz > 62) */
+  int gs_rows;  /* y-rows per workgroup: 1..8 (one wavefront per row; two and at most
+                   4 rows when nz > 62) */
+  int gs_chunk; /* steps whose coefficients a wavefront loads in one batch: 2 or 4 */
   int reserved[2];
 } asam_tr_tuning;
 

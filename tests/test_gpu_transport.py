@@ -60,8 +60,8 @@ def test_matvec_equals_oracle(shape, trm, orc):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("tune", [dict(gs_rows=8, gs_chunk=8), dict(gs_rows=1, gs_chunk=4),
-                                  dict(gs_rows=3, gs_chunk=8), dict(gs_rows=5, gs_chunk=4)])
+@pytest.mark.parametrize("tune", [dict(gs_rows=8, gs_chunk=4), dict(gs_rows=1, gs_chunk=2),
+                                  dict(gs_rows=3, gs_chunk=4), dict(gs_rows=5, gs_chunk=2)])
 def test_psolve_equals_oracle(shape, tune, trm, orc):
     tp = problem(*shape, ncomp=3, fall_comp=1, mom_comps=(2,))
     s = trm.TransportSolver.from_problem(tp)
