@@ -226,8 +226,17 @@ void launch_mass(hipStream_t s, const Geom& g, const double* vol, double* y, con
 __global__ __launch_bounds__(1024) void k_finalize(int kind, int nq, int nblk, int nblk_hi,
                                                    const double* partial, KState* st, int k,
                                                    int i, double* out) {
+  // The scalar step runs on an LDS copy of the solver state: one cooperative load and one
+  // cooperative store instead of a chain of dependent global accesses by one thread (the
+  // state was last written by the previous tail, usually from another XCD's L2).
+  __shared__ KState sk;
   __shared__ double sh[4][16];
+  static_assert(sizeof(KState) % 8 == 0, "KState is copied in 8-byte words");
+  constexpr int kWords = (int)(sizeof(KState) / 8);
   const int tid = threadIdx.x;
+  for (int w = tid; w < kWords; w += 1024)
+    reinterpret_cast<unsigned long long*>(&sk)[w] =
+        reinterpret_cast<const unsigned long long*>(st)[w];
   double v[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -246,14 +255,19 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, int nq, int nblk, i
     for (int q = 0; q < 4; ++q) sh[q][tid >> 6] = v[q];
   }
   __syncthreads();
-  if (tid != 0) return;
-  double S[4];
-  for (int q = 0; q < 4; ++q) {
-    double s = 0.0;
-    for (int w = 0; w < 16; ++w) s += sh[q][w];
-    S[q] = q < nq ? s : 0.0;
+  if (tid == 0) {
+    double S[4];
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < 16; ++w) s += sh[q][w];
+      S[q] = q < nq ? s : 0.0;
+    }
+    fin_logic(kind, S, &sk, k, i, nq, out);
   }
-  fin_logic(kind, S, st, k, i, nq, out);
+  __syncthreads();
+  for (int w = tid; w < kWords; w += 1024)
+    reinterpret_cast<unsigned long long*>(st)[w] =
+        reinterpret_cast<const unsigned long long*>(&sk)[w];
 }
 
 void launch_finalize(hipStream_t s, int kind, int nq, int nblk, const double* partial,

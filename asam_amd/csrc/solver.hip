@@ -49,6 +49,7 @@ struct asam_sound_ctx {
   int counts[8] = {0};
   int last_iters = 4;     // BiCGStab iterations of the last plain solve (first chunk size)
   int inner_hist[32] = {0};  // inner iterations per refinement step of the last mixed solve
+  int mixed_steps = 0;       // refinement steps of the last converged mixed solve (0: unknown)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   // live per-kernel timing (asam_sound_profile_enable)
@@ -652,6 +653,15 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, 
 }
 
 // FP32-inner BiCGStab with FP64 iterative refinement.
+//
+// Refinement step k is the FP64 residual of x_k (with x_k = x_{k-1} + d for k > 0) and,
+// unless it meets the tolerance, an FP32 inner solve of A d = r.  x_k lives in
+// bufs[k & 1]; the device records the parity of the last step (xpar).  When the previous
+// solve of this context left its per-step inner iteration counts, the whole schedule of
+// that many steps is enqueued at once and synchronised once: kernels after convergence
+// return at once on the device flags, and an inner solve the schedule cuts short only
+// makes the next refinement step start from a less refined x.  Whatever is not finished
+// then continues step by step.
 static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, double tol,
                        asam_sound_info* info) {
   Enq e(c, tr);
@@ -663,39 +673,65 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   int rc = upload_state(c);
   if (rc) return rc;
   KState* st = c->st;
-  double* xcur = x;
-  double* xnext = (x == c->xalt) ? c->xd : c->xalt;
-  int inner_total = 0, outer = 0;
-  bool converged = false;
-  for (outer = 0; outer <= c->ctl.max_outer; ++outer) {
-    if (outer == 0) {
-      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, nullptr, nullptr,
-              0, FIN_MIXED_OUTER, 1);
-    } else {
-      e.apply(0, 0, 1, 0, xcur, c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, c->d32, xnext, 0,
-              FIN_MIXED_OUTER, 1);
-      double* tmp = xcur; xcur = xnext; xnext = tmp;
-    }
-    rc = sync_state(c);
-    if (rc) return rc;
-    if (hs->outer_done) { converged = true; break; }
-    if (outer == c->ctl.max_outer) break;
+  double* const bufs[2] = {x, (x == c->xalt) ? c->xd : c->xalt};
+  auto outer_step = [&](int k) {
+    if (k == 0)
+      e.apply(0, 0, 1, 0, bufs[0], c->r32, b, c->rt32, nullptr, DOT_YY, nullptr, nullptr,
+              nullptr, 0, FIN_MIXED_OUTER, 1);
+    else
+      e.apply(0, 0, 1, 0, bufs[(k - 1) & 1], c->r32, b, c->rt32, nullptr, DOT_YY,
+              &st->outer_done, c->d32, bufs[k & 1], 0, FIN_MIXED_OUTER, 1);
+  };
+  auto inner_fill = [&] {
     e.timed(5, 4.0 * c->n, [&] { launch_fill(e.s, c->n, 1, c->d32, 0.0); });
+  };
+  int k = 0;  // last refinement step enqueued
+  const int ns = c->mixed_steps;
+  if (!c->prof_on && ns > 0 && ns <= 16 && ns <= c->ctl.max_outer) {
+    outer_step(0);
+    for (k = 1; k <= ns; ++k) {
+      inner_fill();
+      const int h = c->inner_hist[k - 1] > 0 ? c->inner_hist[k - 1] : 1;
+      for (int q = 0; q < h && q < c->ctl.max_inner; ++q) {
+        e.iter = q + 1;
+        bicg_iteration<float>(e, 1, c->d32, c->r32, c->rt32, c->p32, c->ph32, c->v32, c->rh32,
+                              c->t32);
+      }
+      e.iter = 0;
+      outer_step(k);
+    }
+    k = ns;
+  } else {
+    outer_step(0);
+  }
+  rc = sync_state(c);
+  if (rc) return rc;
+  bool converged = hs->outer_done != 0;
+  // step-by-step continuation: k is the device's current step (its residual is known)
+  k = hs->outer_done ? hs->outer : hs->outer - 1;
+  while (!converged && k < c->ctl.max_outer) {
+    inner_fill();
     int issued = 0;
-    const int hist = outer < 32 ? c->inner_hist[outer] : 0;
+    const int hist = k < 32 ? c->inner_hist[k] : 0;
     rc = bicg_loop<float>(e, 1, c->ctl.max_inner, c->d32, c->r32, c->rt32, c->p32, c->ph32,
                           c->v32, c->rh32, c->t32, &issued, hist > 0 ? hist : 4);
     if (rc) return rc;
-    int res = hs->result;
-    int it = res < 0 ? -res : (hs->status == 1 ? res / 2 : res);
-    if (outer < 32) c->inner_hist[outer] = it > 0 ? it : 1;
-    inner_total += it;
+    ++k;
+    outer_step(k);
+    rc = sync_state(c);
+    if (rc) return rc;
+    converged = hs->outer_done != 0;
   }
-  if (xcur != x) HIPCHK(c, hipMemcpyAsync(x, xcur, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  // per-step inner counts as the device recorded them, for the next solve's schedule
+  const int steps = hs->outer;
+  for (int q = 0; q < steps && q < 16; ++q) c->inner_hist[q] = hs->hist[q] > 0 ? hs->hist[q] : 1;
+  c->mixed_steps = converged ? steps : 0;
+  double* xfin = bufs[hs->xpar & 1];
+  if (xfin != x) HIPCHK(c, hipMemcpyAsync(x, xfin, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (info) {
-    info->iterations = inner_total;
-    info->restarts = outer;
+    info->iterations = hs->inner_total;
+    info->restarts = steps;
     info->status = converged ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
     info->abs_residual = hs->outer_r;
     info->rel_residual = hs->outer_r0 > 0 ? hs->outer_r / hs->outer_r0 : 0.0;

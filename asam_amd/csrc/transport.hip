@@ -669,16 +669,17 @@ extern "C" int asam_tr_set_components(asam_tr_ctx* c, const int* op_mv, const in
 // ------------------------------------------------------------------ launchers
 static bool uses(const Ops& o, int nc, int op) {
   for (int q = 0; q < nc; ++q)
-    if (o.op[q] == op || (op == ASAM_TR_POT && o.op[q] == ASAM_TR_FALL)) return true;
+    if (o.op[q] == op) return true;
   return false;
 }
 
-static int check_ready(asam_tr_ctx* c, bool psolve) {
-  const Ops& o = psolve ? c->ops_ps : c->ops_mv;
+// The matrices a component table needs have been set (JacTPot always: the first
+// residual of BiCGStab takes it for every component).
+static int check_ready(asam_tr_ctx* c, const Ops& o) {
   if (!c->have_pot) return trfail(c, ASAM_E_STATE, "JacTPot not set");
-  if ((uses(o, c->nc, ASAM_TR_MOM) || uses(c->ops_mv, c->nc, ASAM_TR_MOM)) && !c->have_mom)
+  if (uses(o, c->nc, ASAM_TR_MOM) && !c->have_mom)
     return trfail(c, ASAM_E_STATE, "JacTMom not set");
-  if ((uses(o, c->nc, ASAM_TR_FALL) || uses(c->ops_mv, c->nc, ASAM_TR_FALL)) && !c->have_fall)
+  if (uses(o, c->nc, ASAM_TR_FALL) && !c->have_fall)
     return trfail(c, ASAM_E_STATE, "JacFall not set");
   return 0;
 }
@@ -916,7 +917,7 @@ static int solve_dev(asam_tr_ctx* c, double* x, const double* b, int* maxiter, d
 
 extern "C" int asam_tr_matvec_device(asam_tr_ctx* c, const double* x, double* y, int init) {
   if (!c || !x || !y || x == y) return trfail(c, ASAM_E_INVALID, "x, y must be distinct buffers");
-  TRY(check_ready(c, false));
+  TRY(check_ready(c, c->ops_mv));
   TRCHK(c, hipSetDevice(c->device));
   TRY(launch_matvec(c, x, y, init));
   TRCHK(c, hipStreamSynchronize(c->stream));
@@ -925,7 +926,7 @@ extern "C" int asam_tr_matvec_device(asam_tr_ctx* c, const double* x, double* y,
 
 extern "C" int asam_tr_psolve_device(asam_tr_ctx* c, const double* x, double* y) {
   if (!c || !x || !y) return trfail(c, ASAM_E_INVALID, "null argument");
-  TRY(check_ready(c, true));
+  TRY(check_ready(c, c->ops_ps));
   TRCHK(c, hipSetDevice(c->device));
   TRY(launch_psolve(c, x, y));
   return check_gs_error(c);
@@ -936,8 +937,8 @@ extern "C" int asam_tr_solve_device(asam_tr_ctx* c, double* x, const double* b, 
   if (!c || !x || !b || !maxiter || !tol) return trfail(c, ASAM_E_INVALID, "null argument");
   if (x == b) return trfail(c, ASAM_E_INVALID, "x and b must be distinct buffers");
   if (!(*tol >= 0.0)) return trfail(c, ASAM_E_INVALID, "tol");
-  TRY(check_ready(c, true));
-  TRY(check_ready(c, false));
+  TRY(check_ready(c, c->ops_ps));
+  TRY(check_ready(c, c->ops_mv));
   TRCHK(c, hipSetDevice(c->device));
   return solve_dev(c, x, b, maxiter, tol, info);
 }
@@ -946,7 +947,7 @@ static size_t vbytes(const asam_tr_ctx* c) { return sizeof(double) * c->nc * c->
 
 extern "C" int asam_tr_matvec(asam_tr_ctx* c, const double* x, double* y, int init) {
   if (!c || !x || !y) return trfail(c, ASAM_E_INVALID, "null argument");
-  TRY(check_ready(c, false));
+  TRY(check_ready(c, c->ops_mv));
   TRCHK(c, hipSetDevice(c->device));
   TRCHK(c, hipMemcpyAsync(c->vh, x, vbytes(c), hipMemcpyHostToDevice, c->stream));
   TRY(launch_matvec(c, c->vh, c->vt, init));
@@ -957,7 +958,7 @@ extern "C" int asam_tr_matvec(asam_tr_ctx* c, const double* x, double* y, int in
 
 extern "C" int asam_tr_psolve(asam_tr_ctx* c, const double* x, double* y) {
   if (!c || !x || !y) return trfail(c, ASAM_E_INVALID, "null argument");
-  TRY(check_ready(c, true));
+  TRY(check_ready(c, c->ops_ps));
   TRCHK(c, hipSetDevice(c->device));
   TRCHK(c, hipMemcpyAsync(c->vh, x, vbytes(c), hipMemcpyHostToDevice, c->stream));
   TRY(launch_psolve(c, c->vh, c->vt));
@@ -970,8 +971,8 @@ extern "C" int asam_tr_solve(asam_tr_ctx* c, double* x, const double* b, int* ma
                              double* tol, asam_tr_info* info) {
   if (!c || !x || !b || !maxiter || !tol) return trfail(c, ASAM_E_INVALID, "null argument");
   if (!(*tol >= 0.0)) return trfail(c, ASAM_E_INVALID, "tol");
-  TRY(check_ready(c, true));
-  TRY(check_ready(c, false));
+  TRY(check_ready(c, c->ops_ps));
+  TRY(check_ready(c, c->ops_mv));
   TRCHK(c, hipSetDevice(c->device));
   TRCHK(c, hipMemcpyAsync(c->vx, x, vbytes(c), hipMemcpyHostToDevice, c->stream));
   TRCHK(c, hipMemcpyAsync(c->vb, b, vbytes(c), hipMemcpyHostToDevice, c->stream));
@@ -999,7 +1000,7 @@ extern "C" int asam_tr_set_tuning(asam_tr_ctx* c, const asam_tr_tuning* t) {
 
 extern "C" int asam_tr_time_kernel(asam_tr_ctx* c, int which, int reps, double* ms) {
   if (!c || !ms || reps < 1 || which < 0 || which > 2) return trfail(c, ASAM_E_INVALID, "argument");
-  TRY(check_ready(c, which != 1));
+  TRY(check_ready(c, which == 1 ? c->ops_mv : c->ops_ps));
   TRCHK(c, hipSetDevice(c->device));
   hipEvent_t e0, e1;
   TRCHK(c, hipEventCreate(&e0));
