@@ -222,20 +222,31 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
   a.bxw = g.tune.apply_bx; a.byw = 256 / g.tune.apply_bx; a.xcd = g.tune.apply_xcd;
   a.x = (const VX*)r.x; a.dx = r.dx; a.xw = r.xw; a.b = r.b; a.y = (VO*)r.y;
   a.y2 = (VO*)r.y2; a.u = (const VU*)r.u; a.partial = r.partial; a.done = r.done;
-  if (g.tune.apply_variant == 0)
+  a.dead = r.x0_dead;
+  if constexpr ((FL & AF_X0) != 0) {
+    // same grid and partial-sum slots in either variant
     hipLaunchKernelGGL((k_apply_march<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
                        dim3(a.bxw, a.byw), 0, s, a);
-  else
+  } else if (g.tune.apply_variant == 0) {
+    hipLaunchKernelGGL((k_apply_march<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
+                       dim3(a.bxw, a.byw), 0, s, a);
+  } else {
     hipLaunchKernelGGL((k_apply<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks), dim3(a.bxw, a.byw),
                        0, s, a);
+  }
 }
 
 // The fused-option combinations the solvers use.
 template <int NP, int TR, typename CT, typename VX, typename VO, typename VU>
 static void apply_fl(hipStream_t s, const Geom& g, const Coef<CT>& c, const ApplyReq& r) {
   int fl = (r.b ? AF_B : 0) | (r.y2 ? AF_Y2 : 0) | (r.dx ? AF_DX : 0) |
-           ((r.dots & DOT_YY) ? AF_YY : 0) | ((r.dots & DOT_YU) ? AF_YU : 0);
+           ((r.dots & DOT_YY) ? AF_YY : 0) | ((r.dots & DOT_YU) ? AF_YU : 0) |
+           (r.x0_dead ? AF_X0 : 0);
   switch (fl) {
+    case AF_X0 | AF_B | AF_Y2 | AF_YY:
+      if constexpr (sizeof(CT) == 8)
+        apply_t<NP, TR, AF_X0 | AF_B | AF_Y2 | AF_YY, CT, VX, VO, VU>(s, g, c, r);
+      break;
     case 0: apply_t<NP, TR, 0, CT, VX, VO, VU>(s, g, c, r); break;
     case AF_YU: apply_t<NP, TR, AF_YU, CT, VX, VO, VU>(s, g, c, r); break;
     case AF_YY | AF_YU: apply_t<NP, TR, AF_YY | AF_YU, CT, VX, VO, VU>(s, g, c, r); break;

@@ -19,8 +19,9 @@ namespace asam {
 // ===================================================================== apply
 // Fused options of one apply launch (compile-time, so no branch separates the loads
 // of a cell): residual form y = b - Op(x) (AF_B), copy of y (AF_Y2), x_eff = x + dx
-// written to xw (AF_DX), dots y.y (AF_YY) and y.u (AF_YU).
-enum { AF_B = 1, AF_Y2 = 2, AF_DX = 4, AF_YY = 8, AF_YU = 16 };
+// written to xw (AF_DX), dots y.y (AF_YY) and y.u (AF_YU); AF_X0: x is zero, so the
+// residual form is y = b outside the dead cells (no x or coefficient loads).
+enum { AF_B = 1, AF_Y2 = 2, AF_DX = 4, AF_YY = 8, AF_YU = 16, AF_X0 = 32 };
 
 template <typename CT, typename VX, typename VO, typename VU>
 struct KApply {
@@ -38,6 +39,7 @@ struct KApply {
   const VU* u;
   double* partial;
   const int* done;
+  const unsigned char* dead;  // AF_X0: cells whose residual is 0
 };
 
 // Loads of arrays each cell reads exactly once (no neighbour reuse): nontemporal, so they
@@ -81,6 +83,30 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
     const int k0 = kt * a.kp, k1 = min(nz, k0 + a.kp);
     if (i < nx && j < ny) {
       const long cb = i + sy * j;
+      if constexpr ((FL & AF_X0) != 0) {
+        // x = 0: every stencil term is a signed zero, so b + s equals b (a zero b may
+        // change sign); the walk and the dot accumulation order are those of the full
+        // apply below, so the partial sums are the same bits
+        static_assert((FL & AF_B) != 0 && (FL & (AF_DX | AF_YU)) == 0, "x = 0 residual form");
+        const unsigned char* __restrict__ D = a.dead;
+        long c = cb + sz * k0;
+        for (int k = k0; k < k1; ++k, c += sz) {
+          const bool dead = D[c] != 0;
+          if constexpr (NP == 1) {
+            const double yv = dead ? 0.0 : Bv[c];
+            Y[c] = (VO)yv;
+            if constexpr ((FL & AF_Y2) != 0) Y2[c] = (VO)yv;
+            if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
+          } else {
+            const double2 b0 = *reinterpret_cast<const double2*>(Bv + 2 * c);
+            const double yv1 = dead ? 0.0 : b0.x, yv2 = dead ? 0.0 : b0.y;
+            st2(Y, c, yv1, yv2);
+            if constexpr ((FL & AF_Y2) != 0) st2(Y2, c, yv1, yv2);
+            if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+          }
+        }
+        return;
+      }
       const long dW = i > 0 ? -1 : 0, dE = i + 1 < nx ? 1 : 0;
       const long dS = j > 0 ? -sy : 0, dN = j + 1 < ny ? sy : 0;
       if constexpr (NP == 1) {

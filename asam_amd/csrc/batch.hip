@@ -245,7 +245,7 @@ __global__ void k_ens_init(KState* st, int nmem, double tol, double inner_tol, i
   s->half = 0; s->result = 0; s->first = 0; s->pad = 0;
   s->outer_r0 = 0.0; s->outer_r = 0.0; s->outer_tol = tol; s->inner_tol = inner_tol;
   s->outer = 0; s->inner_total = 0; s->outer_done = 0; s->pad2 = 0;
-  s->xpar = 0; s->pad3 = 0;
+  s->xpar = 0; s->pad3 = 0; s->xnz = 0; s->xz = 0;
   for (int q = 0; q < 16; ++q) s->hist[q] = 0;
 }
 

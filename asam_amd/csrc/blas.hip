@@ -185,6 +185,24 @@ void launch_fill(hipStream_t s, long n, int vt, void* x, double v) {
                           v);
 }
 
+// Any entry != 0 (a signed zero counts as zero): the lanes that see one store the flags;
+// they start as xnz = 0, xz = 1 and are only ever set to the same values.
+__global__ __launch_bounds__(256) void k_any_nonzero(long n, const double* __restrict__ x,
+                                                     int* xnz, int* xz) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  bool nz = false;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride)
+    nz |= x[e] != 0.0;
+  if (nz) {
+    *xnz = 1;
+    *xz = 0;
+  }
+}
+
+void launch_any_nonzero(hipStream_t s, long n, const double* x, int* xnz, int* xz) {
+  hipLaunchKernelGGL(k_any_nonzero, dim3(blas_blocks(n)), dim3(256), 0, s, n, x, xnz, xz);
+}
+
 template <typename TD, typename TS>
 __global__ void k_copy_cvt(long n, TD* d, const TS* s) {
   const long stride = (long)gridDim.x * blockDim.x;

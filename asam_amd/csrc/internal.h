@@ -81,6 +81,8 @@ struct KState {
   int outer, inner_total, outer_done, pad2;
   int xpar;             // ensemble refinement: which of the two x buffers holds x
   int pad3;
+  int xnz, xz;          // initial guess has a nonzero entry / is all zero (k_any_nonzero)
+  int pad4, pad5;
   int hist[16];         // inner iterations of refinement step q (q < 16)
 };
 
@@ -115,8 +117,14 @@ struct ApplyReq {
   int dots = 0;
   double* partial = nullptr;
   const int* done = nullptr;
+  // x known to be zero (x unused): the residual form is y = b, zero on the cells this
+  // mask marks dead (their operator row is empty), the same values as the full apply
+  const unsigned char* x0_dead = nullptr;
 };
 int apply_blocks(const Geom& g);
+// dead[c] = 1 where the residual form writes 0 (no volume and no face coefficient), from
+// the FP64 coefficients with the apply's own test.
+void launch_dead_mask(hipStream_t s, const Geom& g, const Coef<double>& c64, unsigned char* dead);
 void launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
                   const ApplyReq& r);
 
@@ -146,6 +154,8 @@ void launch_line(hipStream_t s, const Geom& g, const Coef<double>& c64, const Co
 
 // ------------------------------------------------------------------ BLAS-1 + reductions
 int blas_blocks(long n);
+// *xnz = 1 and *xz = 0 if x (n doubles) has an entry != 0; both untouched otherwise.
+void launch_any_nonzero(hipStream_t s, long n, const double* x, int* xnz, int* xz);
 // BiCGStab tail: x += alpha pHat + omega rHat; r -= omega t; q0 = r.r, q1 = rt.r.
 // With st->half set: x += alpha pHat only.
 void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,

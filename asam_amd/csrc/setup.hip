@@ -261,6 +261,23 @@ void launch_convert(hipStream_t s, const Geom& g, const Coef<double>& c64, Coef<
   cv(c64.pl, c32.pl, P); cv(c64.pt, c32.pt, P); cv(c64.pw, c32.pw, P);
 }
 
+// Dead cells of the residual form: the apply's own test (march_tile), same operand order.
+__global__ void k_dead_mask(int nx, int ny, int nz, const double* kx, const double* ky,
+                            const double* kz, const double* vol, unsigned char* dead) {
+  const long N = (long)nx * ny * nz, sy = nx, sz = (long)nx * ny;
+  const long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / sz);
+  const long mW = i > 0 ? c - 1 : c, mS = j > 0 ? c - sy : c;
+  const double kzm = k > 0 ? kz[c - sz] : 0.0;
+  dead[c] = (vol[c] == 0.0 && kx[mW] + kx[c] + ky[mS] + ky[c] + kzm + kz[c] == 0.0) ? 1 : 0;
+}
+
+void launch_dead_mask(hipStream_t s, const Geom& g, const Coef<double>& c64, unsigned char* dead) {
+  hipLaunchKernelGGL(k_dead_mask, dim3((g.N + 255) / 256), dim3(256), 0, s, g.nx, g.ny, g.nz,
+                     c64.kx, c64.ky, c64.kz, c64.vol, dead);
+}
+
 // ===================================================================== theta VJP
 // d(mu^T A x)/d theta_c through the face theta th_f = (th_a V_a + th_b V_b)/(V_a+V_b+Eps).
 // MultiTriT: dS/dth_f = (k_f/th_f) (q_a - q_b)(mu_a - mu_b)
