@@ -247,6 +247,13 @@ static void apply_fl(hipStream_t s, const Geom& g, const Coef<CT>& c, const Appl
       if constexpr (sizeof(CT) == 8)
         apply_t<NP, TR, AF_X0 | AF_B | AF_Y2 | AF_YY, CT, VX, VO, VU>(s, g, c, r);
       break;
+    case AF_X0 | AF_B | AF_YY:
+      if constexpr (sizeof(CT) == 8)
+        apply_t<NP, TR, AF_X0 | AF_B | AF_YY, CT, VX, VO, VU>(s, g, c, r);
+      break;
+    case AF_X0 | AF_B:
+      if constexpr (sizeof(CT) == 8) apply_t<NP, TR, AF_X0 | AF_B, CT, VX, VO, VU>(s, g, c, r);
+      break;
     case 0: apply_t<NP, TR, 0, CT, VX, VO, VU>(s, g, c, r); break;
     case AF_YU: apply_t<NP, TR, AF_YU, CT, VX, VO, VU>(s, g, c, r); break;
     case AF_YY | AF_YU: apply_t<NP, TR, AF_YY | AF_YU, CT, VX, VO, VU>(s, g, c, r); break;

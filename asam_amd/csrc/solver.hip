@@ -416,21 +416,23 @@ struct Enq {
   // Unprofiled, a scan of x picks on the device which of two launches writes y: the full
   // FP64 apply, or, for an all-zero x (the usual first guess), the x = 0 form that reads
   // only b and the dead-cell mask (same values and partial sums, a quarter of the bytes).
-  // Needs st->xnz = 0 and st->xz = 1 in the uploaded state.
-  void resid_init(int vo, const double* x, void* y, const double* b, void* y2, int fkind) {
+  // Needs st->xnz = 0 and st->xz = 1 in the uploaded state.  dots: DOT_YY or 0; fkind < 0:
+  // no finalize.
+  void resid_init(int vo, const double* x, void* y, const double* b, void* y2, int dots,
+                  int fkind) {
     if (c->prof_on) {
-      apply(0, 0, vo, 0, x, y, b, y2, nullptr, DOT_YY, nullptr, nullptr, nullptr, 0, fkind, 1);
+      apply(0, 0, vo, 0, x, y, b, y2, nullptr, dots, nullptr, nullptr, nullptr, 0, fkind, 1);
       return;
     }
     launch_any_nonzero(s, c->n, x, &c->st->xnz, &c->st->xz);
     ApplyReq r;
     r.tr = tr; r.prec = 0; r.vx = 0; r.vo = vo; r.vu = 0; r.x = x; r.y = y; r.b = b;
-    r.y2 = y2; r.dots = DOT_YY; r.partial = c->partial; r.done = &c->st->xz;
+    r.y2 = y2; r.dots = dots; r.partial = c->partial; r.done = &c->st->xz;
     launch_apply(s, c->g, c->c64, c->c32, r);
     r.x = nullptr; r.x0_dead = c->dead; r.done = &c->st->xnz;
     launch_apply(s, c->g, c->c64, c->c32, r);
     c->counts[0]++;
-    fin(fkind, 1, apply_blocks(c->g));
+    if (fkind >= 0) fin(fkind, 1, apply_blocks(c->g));
   }
   void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
             const double* scale, const int* done, int qslot = 0) {
@@ -564,7 +566,7 @@ static int solve_bicg_fp64(asam_sound_ctx* c, int tr, double* x, const double* b
   int rc = upload_state(c);
   if (rc) return rc;
   // r = b - MatVec(x); rTilde = r; rNorm0 = |r|
-  e.resid_init(0, x, c->r, b, c->rt, FIN_BICG_INIT);
+  e.resid_init(0, x, c->r, b, c->rt, DOT_YY, FIN_BICG_INIT);
   int issued = 0;
   if (*maxiter > 0) {
     e.iter = 0;
@@ -619,17 +621,25 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, 
   hs->m = m;
   hs->gtol = tol;
   hs->maxcycles = maxcycles;
+  hs->xz = 1;
   rc = upload_state(c);
   if (rc) return rc;
   KState* st = c->st;
   const Geom& g = c->g;
   const long n = c->n;
   for (int cyc = 0; cyc < maxcycles; ++cyc) {
+    // the first cycle's residual may start from a zero guess; later ones from the iterate
     if (flexible) {
-      e.apply(0, 0, 0, 0, x, c->gv[0], b, nullptr, nullptr, DOT_YY, &st->done);
+      if (cyc == 0)
+        e.resid_init(0, x, c->gv[0], b, nullptr, DOT_YY, -1);
+      else
+        e.apply(0, 0, 0, 0, x, c->gv[0], b, nullptr, nullptr, DOT_YY, &st->done);
       e.fin(FIN_GM_INIT, 1, apply_blocks(g));
     } else {
-      e.apply(0, 0, 0, 0, x, c->gtmp, b, nullptr, nullptr, 0, &st->done);
+      if (cyc == 0)
+        e.resid_init(0, x, c->gtmp, b, nullptr, 0, -1);
+      else
+        e.apply(0, 0, 0, 0, x, c->gtmp, b, nullptr, nullptr, 0, &st->done);
       e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[0], nullptr, &st->done);
       e.mgs(0, c->gv[0], nullptr, nullptr, 0, -1, 1);
       e.fin(FIN_GM_INIT, 1, blas_blocks(n));
@@ -700,7 +710,7 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   double* const bufs[2] = {x, (x == c->xalt) ? c->xd : c->xalt};
   auto outer_step = [&](int k) {
     if (k == 0)
-      e.resid_init(1, bufs[0], c->r32, b, c->rt32, FIN_MIXED_OUTER);
+      e.resid_init(1, bufs[0], c->r32, b, c->rt32, DOT_YY, FIN_MIXED_OUTER);
     else
       e.apply(0, 0, 1, 0, bufs[(k - 1) & 1], c->r32, b, c->rt32, nullptr, DOT_YY,
               &st->outer_done, c->d32, bufs[k & 1], 0, FIN_MIXED_OUTER, 1);

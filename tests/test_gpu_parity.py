@@ -146,25 +146,27 @@ def test_mixed_precision_reaches_1e8(np1, oracle_mod, sound_mod):
     assert err < 1e-6
 
 
-@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
+@pytest.mark.parametrize("method,precision", [("BICGSTAB", "FP64"), ("BICGSTAB", "MIXED"),
+                                              ("FGMRES", "FP64"), ("GMRES", "FP64")])
 @pytest.mark.parametrize("np1", [1, 2])
-def test_zero_guess_residual_form(np1, precision, oracle_mod, sound_mod):
+def test_zero_guess_residual_form(np1, method, precision, oracle_mod, sound_mod):
     """The initial residual of an all-zero guess comes from the x = 0 form (b outside the
     dead cells); with profiling on, the full apply computes it.  Both give the same bits,
     and a nonzero guess takes the full apply in either mode."""
     p = W.config(4, np1=np1, scale=0.5)
     dead = int(np.sum(p.VolC == 0.0))
     x1 = live(p, 1e-3 * W.normal(p.n, 5))
+    maxiter = 300 if method == "BICGSTAB" else 20
     for x0 in (np.zeros(p.n), x1):
         out = []
         for prof in (False, True):
             s = sound_mod.SoundSolver.from_problem(p)
-            s.set_control(precision=precision)
+            s.set_control(method=method, precision=precision)
             s.profile(prof)
-            out.append(s.SolveSound(x0, p.b, 300, 1e-8))
+            out.append(s.SolveSound(x0, p.b, maxiter, 1e-8))
             s.close()
         (xa, ma, ta, ia), (xb, mb, tb, ib) = out
-        print(f"np1={np1} {precision} guess {'zero' if not x0.any() else 'nonzero'} "
+        print(f"np1={np1} {method} {precision} guess {'zero' if not x0.any() else 'nonzero'} "
               f"({dead} dead cells): iterations {ia.iterations}/{ib.iterations}, "
               f"bitwise {np.array_equal(xa, xb)}")
         assert np.array_equal(xa, xb) and ma == mb and ta == tb
