@@ -63,11 +63,14 @@ struct KEnsApply {
   long pstride;
   const KState* st;
   int skip_outer;   // skip members with outer_done (else members with done)
+  const unsigned char* dead = nullptr;  // AF_X0: cells whose residual is 0 (all members)
+  const int* skip = nullptr;            // the whole launch returns at once when *skip != 0
 };
 
 // PAR: refinement apply x_eff = X[(o-1)&1] + d written to X[o&1], o = refinement step.
 template <int NP, int FL, int PAR, typename CT, typename VX, typename VO, typename VU>
 __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) {
+  if (e.skip && *e.skip) return;
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int nb8 = gridDim.x >> 3, per = (e.ntiles + 7) >> 3;
   const int g0 = (blockIdx.x & 7) * per, g1 = min(e.ntiles, g0 + per);
@@ -103,6 +106,7 @@ __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) 
     a.u = e.u ? e.u + v : nullptr;
     a.partial = nullptr;
     a.done = nullptr;
+    a.dead = e.dead ? e.dead + M.off : nullptr;
     double acc[2] = {0.0, 0.0};
     march_tile<NP, 0, FL, CT, VX, VO, VU>(a, g - M.at0, acc);
     if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
@@ -235,8 +239,13 @@ __global__ __launch_bounds__(256) void k_ens_xcopy(KEnsVec e, double* __restrict
 }
 
 // Per-member solve state at the start of a solve.
-__global__ void k_ens_init(KState* st, int nmem, double tol, double inner_tol, int max_inner) {
+__global__ void k_ens_init(KState* st, int nmem, double tol, double inner_tol, int max_inner,
+                           int* xflags) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0) {
+    xflags[0] = 0;  // the initial guess has a nonzero entry
+    xflags[1] = 1;  // the initial guess is all zero
+  }
   if (m >= nmem) return;
   KState* s = st + m;
   s->rho = 0.0; s->rho_old = 0.0; s->alpha = 0.0; s->omega = 0.0; s->beta = 0.0;
@@ -323,6 +332,8 @@ struct asam_ens_ctx {
   EnsMember* md = nullptr;
   KState* st = nullptr;
   KState* sth = nullptr;  // pinned
+  unsigned char* dead = nullptr;  // cells whose residual is 0, from the FP64 coefficients
+  int* xflags = nullptr;          // initial guess: [0] has a nonzero entry, [1] is all zero
   asam_sound_control ctl{};
   asam_ens_tuning tune{};
   std::vector<void*> allocs;
@@ -511,10 +522,20 @@ void apply_outer(asam_ens_ctx* c, bool first, double* x, const double* b) {
   e.partial = c->partial; e.pstride = c->pstride; e.st = c->st; e.skip_outer = 1;
   const dim3 grid(grid_for(c->nA, c->tune.apply_pf, c->ncu));
   const dim3 blk(c->tune.apply_bx, 256 / c->tune.apply_bx);
-  if (first)
+  if (first) {
+    // a scan of the initial guess picks on the device which launch writes the residual:
+    // the full FP64 apply, or for an all-zero guess the x = 0 form (b outside the dead
+    // cells: the same values and partial sums from a quarter of the bytes)
+    launch_any_nonzero(c->stream, c->n, x, c->xflags, c->xflags + 1);
+    e.skip = c->xflags + 1;
     hipLaunchKernelGGL((k_ens_apply<NP, AF_B | AF_Y2 | AF_YY, 0, double, double, float, float>),
                        grid, blk, 0, c->stream, e);
-  else
+    e.skip = c->xflags;
+    e.dead = c->dead;
+    hipLaunchKernelGGL(
+        (k_ens_apply<NP, AF_X0 | AF_B | AF_Y2 | AF_YY, 0, double, double, float, float>), grid,
+        blk, 0, c->stream, e);
+  } else
     hipLaunchKernelGGL(
         (k_ens_apply<NP, AF_B | AF_Y2 | AF_DX | AF_YY, 1, double, double, float, float>), grid,
         blk, 0, c->stream, e);
@@ -603,7 +624,7 @@ template <int NP>
 void enqueue_schedule(asam_ens_ctx* c, double* x, const double* b, double tol,
                       const std::vector<int>& sched) {
   hipLaunchKernelGGL(k_ens_init, dim3((c->nmem + 63) / 64), dim3(64), 0, c->stream, c->st,
-                     c->nmem, tol, c->ctl.inner_tol, c->ctl.max_inner);
+                     c->nmem, tol, c->ctl.inner_tol, c->ctl.max_inner, c->xflags);
   outer_step<NP>(c, true, x, b);
   for (size_t q = 0; q < sched.size(); ++q) {
     zero_d(c);
@@ -783,6 +804,8 @@ extern "C" int asam_ens_create(asam_ens_ctx** out, int nmem, const asam_sound_gr
   for (float** q : v32) rc |= ealloc(c, q, c->n);
   rc |= ealloc(c, &c->md, nmem);
   rc |= ealloc(c, &c->st, nmem);
+  rc |= ealloc(c, &c->dead, c->Ntot);
+  rc |= ealloc(c, &c->xflags, 2);
   if (rc) return ASAM_E_NOMEM;
   EHIP(c, hipHostMalloc((void**)&c->sth, sizeof(KState) * nmem, hipHostMallocDefault));
   memset(c->sth, 0, sizeof(KState) * nmem);
@@ -873,6 +896,7 @@ extern "C" int asam_ens_setup(asam_ens_ctx* c, int m, const double* FU, const do
               th + nfu, th + nfu + nfv, k64);
   launch_factor(s, g, k64);
   launch_convert(s, g, k64, k32);
+  launch_dead_mask(s, g, k64, c->dead + M.off);
   EHIP(c, hipGetLastError());
   EHIP(c, hipStreamSynchronize(s));
   double* tmps[] = {dFU, dFV, dFW, dV, ddx, ddy, ddz, drho, drth, dT, tmp, th};
