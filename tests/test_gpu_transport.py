@@ -109,6 +109,29 @@ def test_bicgstab_matches_oracle(shape, met, trm, orc):
             assert abs(t - to) <= 1e-6 * to + 1e-300
 
 
+@pytest.mark.parametrize("tune", [dict(gs_rows=8, gs_chunk=2), dict(gs_rows=2, gs_chunk=4)])
+def test_limits_nz126_16_components(tune, trm, orc):
+    """The largest column (nz = 126: Z = 128, two full wavefronts per row) and the most
+    components (16, every operator kind, Met form): operator and preconditioner bitwise,
+    the solve within 1e-10 of the oracle."""
+    p = W.make_problem(7, 4, 126, 500.0, 500.0, 100.0, H=700.0, a=1200.0,
+                       ridge_xc=0.5 * 7 * 500.0, np1=1, seed=9)
+    tp = W.transport_problem(p, ncomp=16, fall_comp=5, mom_comps=(2, 11), met=True, wind=25.0,
+                             noise=4.0)
+    s = trm.TransportSolver.from_problem(tp)
+    s.set_tuning(**tune)
+    o = orc.TransportOracle(tp)
+    x = W.normal(tp.ncomp * tp.m, 51).reshape(tp.ncomp, tp.m)
+    print(f"tuning {s.get_tuning().as_dict()}", flush=True)
+    assert np.array_equal(s.MatVec(x), o.matvec(x))
+    assert np.array_equal(s.PSolve(x), o.psolve(x))
+    xg, mi, t, info = s.BiCGStab(np.zeros_like(tp.b), tp.b, 60, 1e-12)
+    xo, mio, to, sto = o.bicgstab(tp.b, 60, 1e-12)
+    err = np.linalg.norm(tp.interior(xg) - tp.interior(xo)) / np.linalg.norm(tp.interior(xo))
+    print(f"MaxIter {mi}/{mio}, error vs oracle {err:.2e}")
+    assert info.status == 0 and sto == 0 and abs(abs(mi) - abs(mio)) <= 1 and err <= 1e-10
+
+
 def test_return_conventions(trm, orc):
     tp = problem(16, 12, 10, ncomp=2, wind=25.0, noise=4.0)
     s = trm.TransportSolver.from_problem(tp)
