@@ -440,9 +440,9 @@ def main():
     a = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: the sound stage solve is not sharded across GPUs "
-              "in this build (see DESIGN.md for the horizontal-tile sharding plan).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the sound stage solve runs on a single GPU and is "
+              "not sharded (a sharded run would give each GPU horizontal tiles of full "
+              "columns); run it with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if a.kernel_probe:
         kernel_probe(a.kernel_probe, a.np1, a.reps)
