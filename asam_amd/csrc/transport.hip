@@ -244,26 +244,69 @@ __device__ __forceinline__ bool is_sentinel(double v) {
   return (unsigned long long)__double_as_longlong(v) == kSentinel;
 }
 
+// Waits until the boundary values of C steps from the previous block are all published
+// (not the sentinel), re-reading the ones that are not; bounded, and kept out of line so
+// the rare wait does not bloat the step code.  ok = false when it gave up.
+template <int C>
+struct Bound {
+  double v[C];
+  bool ok;
+};
+template <int C>
+__device__ __attribute__((noinline)) Bound<C> wait_boundary(Bound<C> by, const double* bnd_in,
+                                                          int T0, int j, int iz, int izc, int S,
+                                                          int Z, int X, int* err) {
+  int n = 0;
+  by.ok = true;
+  while (true) {
+    bool miss = false;
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int sl = T0 + q - j, ix = sl - iz;
+      const bool need = sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
+      miss |= need && is_sentinel(by.v[q]);
+    }
+    if (!__any(miss)) return by;
+    if (++n > kSpinLimit) {
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      by.ok = false;
+      return by;
+    }
+    __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+    for (int q = 0; q < C; ++q) {
+      const int sl = T0 + q - j;
+      const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
+      if (is_sentinel(by.v[q]))
+        by.v[q] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 // Forward substitution (L+D) x = t, pipelined over rows.  A row is Q = ceil(Z/64)
 // wavefronts, lane iz = 64*qz + lane; step T of the workgroup is step s = T - j of row j
-// (row iy = g*R + j).  Each wavefront loads the t and L/D coefficients of C steps at once,
-// double-buffered: the loads of the next C steps are in flight while the current C steps
-// run from registers.  New values of row iy-1 come through LDS (one barrier per step) or,
-// for the first row of a block, from the previous block's boundary row; the z-neighbour
-// from the lane below (through LDS across the two wavefronts of a row), the x-neighbour
-// from the lane's own previous step.  Boundary rows are written with agent-scope atomic
-// stores into a buffer pre-filled with a NaN-payload sentinel and read with agent-scope
-// atomic loads; a value that is not the sentinel is final, so no fence or flag orders
-// them.  Blocks take their rows from an atomic ticket and only wait for a block that
-// started before them; every wait is bounded.
+// (row iy = g*R + j).  The steps run in chunks of C; each wavefront keeps the t and L/D
+// coefficients of K chunks in registers and refills a chunk's registers as soon as its
+// steps are done, so the loads of a step are issued (K-1)*C steps before it runs (the
+// loop turns once per K chunks, and the compiler drains the loads in flight there).  New
+// values of row iy-1 come through LDS (one barrier per step) or, for the first row of a
+// block, from the previous block's boundary row, requested one chunk ahead; the
+// z-neighbour from the lane below (through LDS across the two wavefronts of a row), the
+// x-neighbour from the lane's own previous step.  Boundary rows are written with
+// agent-scope atomic stores into a buffer pre-filled with a NaN-payload sentinel and read
+// with agent-scope atomic loads; a value that is not the sentinel is final, so no fence or
+// flag orders them.  Blocks take their rows from an atomic ticket and only wait for a
+// block that started before them; every wait is bounded.
 template <int C>
 struct Chunk {
-  double t[C], w0[C], w1[C], w2[C], w3[C], f0[C], by[C];
+  double t[C], w0[C], w1[C], w2[C], w3[C], f0[C];
 };
 
-template <int C, bool FALL>
+template <int C, int K, bool FALL>
 __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
 #pragma clang fp contract(off)
+  static_assert(K % 2 == 0, "boundary values alternate between two buffers");
   extern __shared__ double lds[];  // [2][R][Z]
   __shared__ int s_ticket;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -294,61 +337,42 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
   double* bnd_out = a.bnd + ((long)c * a.G + g) * SZ;
   const double* bnd_in = a.bnd + ((long)c * a.G + (g > 0 ? g - 1 : 0)) * SZ;
   const int steps = S + R - 1;
+  auto slot = [&](int sl) { return sl < 0 ? 0 : (sl >= S ? S - 1 : sl); };
 
   auto load = [&](int T0, Chunk<C>& ch) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const int sl = T0 + q - j;
-      const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-      const long k = rowbase + (long)slc * Z + izc;
+      const long k = rowbase + (long)slot(T0 + q - j) * Z + izc;
       ch.t[q] = ts[k];
       ch.w0[q] = W[k];
       ch.w1[q] = W[sks + k];
       ch.w2[q] = W[2 * sks + k];
       ch.w3[q] = W[3 * sks + k];
       ch.f0[q] = FALL ? a.wf[k] : 0.0;
-      ch.by[q] = consume ? __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT)
-                         : 0.0;
     }
+  };
+  auto load_by = [&](int T0, Bound<C>& by) {
+#pragma unroll
+    for (int q = 0; q < C; ++q)
+      by.v[q] = consume ? __hip_atomic_load(bnd_in + (long)slot(T0 + q - j) * Z + izc,
+                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                        : 0.0;
   };
   double prev = 0.0;
   bool broken = false;
-  // runs the C steps of ch; the loads of the chunk after (nT0 into nx) are issued once
-  // its first step is done, so that step's waits do not include them
-  auto run = [&](int T0, Chunk<C>& ch, int nT0, Chunk<C>& nx) {
+  // runs the C steps from T0 with coefficients ch and boundary values by; the boundary
+  // values of the next chunk (into nby) are requested once the first step is done, and
+  // ch is refilled with the coefficients of the chunk K chunks later after the last step
+  auto run = [&](int T0, Chunk<C>& ch, Bound<C>& by, Bound<C>& nby) {
     // the first row of a block waits until the previous block's last row has all C steps
     if (consume && !broken) {
-      int n = 0;
-      while (true) {
-        bool miss = false;
-#pragma unroll
-        for (int q = 0; q < C; ++q) {
-          const int sl = T0 + q - j, ix = sl - iz;
-          const bool need = sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
-          miss |= need && is_sentinel(ch.by[q]);
-        }
-        if (!__any(miss)) break;
-        if (++n > kSpinLimit) {
-          __hip_atomic_store(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          broken = true;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-#pragma unroll
-        for (int q = 0; q < C; ++q) {
-          const int sl = T0 + q - j;
-          const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-          if (is_sentinel(ch.by[q]))
-            ch.by[q] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-        }
-      }
+      by = wait_boundary<C>(by, bnd_in, T0, j, iz, izc, S, Z, X, a.err);
+      broken = !by.ok;
       // settle the boundary values inside this branch (an empty register constraint, no
       // instruction), so no load of them is still counted as pending where the
       // wavefronts that do not consume join again
 #pragma unroll
-      for (int q = 0; q < C; ++q) asm volatile("" : "+v"(ch.by[q]));
+      for (int q = 0; q < C; ++q) asm volatile("" : "+v"(by.v[q]));
     }
 #pragma unroll
     for (int q = 0; q < C; ++q) {
@@ -365,7 +389,7 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       if (valid) {
         double t = ch.t[q];
         if (iz > 0) t = t - ch.w0[q] * (lane > 0 ? below : xz_l);
-        if (iy > 0) t = t - ch.w1[q] * (j > 0 ? xy_l : ch.by[q]);
+        if (iy > 0) t = t - ch.w1[q] * (j > 0 ? xy_l : by.v[q]);
         if (ix > 0) t = t - ch.w2[q] * prev;
         double dg = ch.w3[q];
         if (FALL) dg = dg + ch.f0[q] * alpha;
@@ -378,14 +402,19 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       if (iz < Z) lds[cb + j * Z + iz] = out;
       prev = out;
       __syncthreads();
-      if (q == 0) load(nT0, nx);
+      if (q == 0) load_by(T0 + C, nby);
     }
+    load(T0 + K * C, ch);
   };
-  Chunk<C> A, B;
-  load(0, A);
-  for (int T0 = 0; T0 < steps; T0 += 2 * C) {
-    run(T0, A, T0 + C, B);
-    if (T0 + C < steps) run(T0 + C, B, T0 + 2 * C, A);
+  Chunk<C> buf[K];
+  Bound<C> byb[2];
+#pragma unroll
+  for (int b = 0; b < K; ++b) load(b * C, buf[b]);
+  load_by(0, byb[0]);
+  for (int T0 = 0; T0 < steps; T0 += K * C) {
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+      if (T0 + b * C < steps) run(T0 + b * C, buf[b], byb[b & 1], byb[(b + 1) & 1]);
   }
 }
 
@@ -709,16 +738,18 @@ static int launch_matvec(asam_tr_ctx* c, const double* x, double* y, int init) {
   return 0;
 }
 
-template <int C, bool FALL>
+template <int C, int K, bool FALL>
 static void lower_launch(asam_tr_ctx* c, const GsArgs& a, int R, int Q) {
   const size_t lds = 2 * (size_t)R * c->d.Z * sizeof(double);
-  k_gs_lower<C, FALL><<<(unsigned)(c->nc * a.G), 64 * R * Q, lds, c->stream>>>(a, c->ops_ps, Q);
+  k_gs_lower<C, K, FALL><<<(unsigned)(c->nc * a.G), 64 * R * Q, lds, c->stream>>>(a, c->ops_ps,
+                                                                                 Q);
 }
 
+// gs_chunk 2: chunks of 2 steps, 6 in registers; 4: chunks of 4 steps, 2 in registers
 template <bool FALL>
 static void lower_kind(asam_tr_ctx* c, const GsArgs& a, int R) {
-  if (c->tune.gs_chunk >= 4) lower_launch<4, FALL>(c, a, R, c->M);
-  else lower_launch<2, FALL>(c, a, R, c->M);
+  if (c->tune.gs_chunk >= 4) lower_launch<4, 2, FALL>(c, a, R, c->M);
+  else lower_launch<2, 6, FALL>(c, a, R, c->M);
 }
 
 template <int NC>
