@@ -244,73 +244,92 @@ __device__ __forceinline__ bool is_sentinel(double v) {
   return (unsigned long long)__double_as_longlong(v) == kSentinel;
 }
 
-// Waits until the boundary values of C steps from the previous block are all published
-// (not the sentinel), re-reading the ones that are not; bounded, and kept out of line so
-// the rare wait does not bloat the step code.  ok = false when it gave up.
+// Boundary values of C steps for the 128 z-slots of a row, two per lane (z = lane and
+// lane + 64), as one polling wavefront holds them.
 template <int C>
-struct Bound {
-  double v[C];
+struct Pend {
+  double v[C][2];
   bool ok;
 };
+
+// Re-reads the boundary values that are still the sentinel until every value the first
+// row will use is published; bounded, and kept out of line so that the rare wait does not
+// bloat the kernel's code.  ok = false when it gave up (the error word is then set).
 template <int C>
-__device__ __attribute__((noinline)) Bound<C> wait_boundary(Bound<C> by, const double* bnd_in,
-                                                          int T0, int j, int iz, int izc, int S,
-                                                          int Z, int X, int* err) {
+__device__ __attribute__((noinline)) Pend<C> settle_boundary(Pend<C> p, const double* bnd_in,
+                                                           int T0, int S, int Z, int X,
+                                                           int* err) {
+  const int lane = threadIdx.x & 63;
   int n = 0;
-  by.ok = true;
+  p.ok = true;
   while (true) {
     bool miss = false;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const int sl = T0 + q - j, ix = sl - iz;
-      const bool need = sl >= 0 && sl < S && iz < Z && ix >= 0 && ix < X;
-      miss |= need && is_sentinel(by.v[q]);
+      const int sl = T0 + q;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int z = lane + 64 * h, ix = sl - z;
+        const bool need = sl >= 0 && sl < S && z < Z && ix >= 0 && ix < X;
+        miss |= need && is_sentinel(p.v[q][h]);
+      }
     }
-    if (!__any(miss)) return by;
+    if (!__any(miss)) return p;
     if (++n > kSpinLimit) {
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      by.ok = false;
-      return by;
+      p.ok = false;
+      return p;
     }
     __builtin_amdgcn_s_sleep(2);
 #pragma unroll
     for (int q = 0; q < C; ++q) {
-      const int sl = T0 + q - j;
-      const int slc = sl < 0 ? 0 : (sl >= S ? S - 1 : sl);
-      if (is_sentinel(by.v[q]))
-        by.v[q] = __hip_atomic_load(bnd_in + (long)slc * Z + izc, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
+      const int sl = T0 + q;
+      const long o = (long)(sl < 0 ? 0 : (sl >= S ? S - 1 : sl)) * Z;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int z = lane + 64 * h;
+        if (is_sentinel(p.v[q][h]))
+          p.v[q][h] = __hip_atomic_load(bnd_in + o + (z < Z ? z : Z - 1), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
   }
 }
 
-// Forward substitution (L+D) x = t, pipelined over rows.  A row is Q = ceil(Z/64)
-// wavefronts, lane iz = 64*qz + lane; step T of the workgroup is step s = T - j of row j
-// (row iy = g*R + j).  The steps run in chunks of C; each wavefront keeps the t and L/D
-// coefficients of K chunks in registers and refills a chunk's registers as soon as its
-// steps are done, so the loads of a step are issued (K-1)*C steps before it runs (the
-// loop turns once per K chunks, and the compiler drains the loads in flight there).  New
-// values of row iy-1 come through LDS (one barrier per step) or, for the first row of a
-// block, from the previous block's boundary row, requested one chunk ahead; the
-// z-neighbour from the lane below (through LDS across the two wavefronts of a row), the
-// x-neighbour from the lane's own previous step.  Boundary rows are written with
-// agent-scope atomic stores into a buffer pre-filled with a NaN-payload sentinel and read
-// with agent-scope atomic loads; a value that is not the sentinel is final, so no fence or
-// flag orders them.  Blocks take their rows from an atomic ticket and only wait for a
-// block that started before them; every wait is bounded.
+// Forward substitution (L+D) x = t, pipelined over rows.  A workgroup is R rows of
+// Q = ceil(Z/64) compute wavefronts (lane iz = 64*qz + lane) plus one polling wavefront;
+// step T of the workgroup is step s = T - j of row j (row iy = g*R + j).
+//  - The steps run in chunks of C.  Each compute wavefront keeps the t and L/D
+//    coefficients of K chunks in registers and refills a chunk's registers as soon as its
+//    steps are done, so the loads of a step are issued (K-1)*C steps before it runs.
+//  - New values of row iy-1 come through LDS (one barrier per step); the z-neighbour from
+//    the lane below (through LDS across the two wavefronts of a row), the x-neighbour
+//    from the lane's own previous step.
+//  - The first row's neighbour row is the previous block's last row.  Its values are
+//    written with agent-scope atomic stores into a buffer pre-filled with a NaN-payload
+//    sentinel; a value that is not the sentinel is final, so no fence or flag orders
+//    them.  The polling wavefront reads them with agent-scope atomic loads, a chunk
+//    ahead and in a wavefront of their own (so no coefficient load is queued ahead of a
+//    poll), and hands them to the first row through LDS.
+//  - Blocks take their rows from an atomic ticket and only wait for a block that started
+//    before them; every wait is bounded.
 template <int C>
 struct Chunk {
   double t[C], w0[C], w1[C], w2[C], w3[C], f0[C];
 };
 
+constexpr int kBoundStride = 128;  // z-stride of the boundary-value slots in LDS
+
 template <int C, int K, bool FALL>
-__global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
+__global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
 #pragma clang fp contract(off)
-  static_assert(K % 2 == 0, "boundary values alternate between two buffers");
-  extern __shared__ double lds[];  // [2][R][Z]
+  extern __shared__ double lds[];  // [2][R][Z] row exchange, then [2][C][128] boundary values
   __shared__ int s_ticket;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int j = wv / Q, qz = wv - j * Q, R = (blockDim.x >> 6) / Q;
+  const int nw = (blockDim.x >> 6) - 1;  // compute wavefronts; the last one polls
+  const bool poller = wv == nw;
+  const int R = nw / Q;
+  const int j = poller ? 0 : wv / Q, qz = poller ? 0 : wv - j * Q;
   const Dims& d = a.d;
   const int c = blockIdx.x / a.G;
   const int op = ops.op[c];
@@ -320,6 +339,73 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
   if (threadIdx.x == 0) s_ticket = atomicAdd(a.counter + c, 1);
   __syncthreads();
   const int g = s_ticket;
+  const int S = d.S, Z = d.Z, X = d.X;
+  const long SZ = (long)S * Z;
+  const int steps = S + R - 1;
+  double* byL = lds + 2 * R * Z;
+  auto slot = [&](int sl) { return sl < 0 ? 0 : (sl >= S ? S - 1 : sl); };
+
+  if (poller) {
+    // every barrier of the compute wavefronts is matched here, one per step
+    const bool active = g > 0;
+    const double* bnd_in = a.bnd + ((long)c * a.G + (g > 0 ? g - 1 : 0)) * SZ;
+    Pend<C> p;
+    bool broken = false;
+    auto issue = [&](int T0) {
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const long o = (long)slot(T0 + q) * Z;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int z = lane + 64 * h;
+          p.v[q][h] = __hip_atomic_load(bnd_in + o + (z < Z ? z : Z - 1), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    };
+    // waits for the values of the chunk at T0 and puts them in LDS slot kc & 1
+    auto settle = [&](int T0, int kc) {
+      bool miss = false;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const int sl = T0 + q;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int z = lane + 64 * h, ix = sl - z;
+          miss |= sl >= 0 && sl < S && z < Z && ix >= 0 && ix < X && is_sentinel(p.v[q][h]);
+        }
+      }
+      if (__any(miss) && !broken) {
+        p = settle_boundary<C>(p, bnd_in, T0, S, Z, X, a.err);
+        broken = !p.ok;
+      }
+#pragma unroll
+      for (int q = 0; q < C; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          byL[((kc & 1) * C + q) * kBoundStride + lane + 64 * h] = p.v[q][h];
+    };
+    if (active) {
+      issue(0);
+      settle(0, 0);
+    }
+    __syncthreads();
+    for (int T0 = 0; T0 < steps; T0 += C) {
+      const int kc = T0 / C;
+#pragma unroll
+      for (int q = 0; q < C; ++q) {
+        const int T = T0 + q;
+        if (T >= steps) break;
+        // the next chunk's values: requested after this chunk's first barrier, in LDS
+        // before its last one
+        if (active && q == 0 && T0 + C < steps) issue(T0 + C);
+        if (active && (q == C - 1 || T == steps - 1) && T0 + C < steps) settle(T0 + C, kc + 1);
+        __syncthreads();
+      }
+    }
+    return;
+  }
+
   const int iy = g * R + j;
   const bool row_ok = iy < d.Y;
   const double alpha = ops.alpha[c];
@@ -327,17 +413,11 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
   const long sks = d.sks;
   const double* __restrict__ ts = a.ts + c * sks;
   double* xs = a.xs + c * sks;
-  const int S = d.S, Z = d.Z, X = d.X;
-  const long SZ = (long)S * Z;
   const long rowbase = (long)(row_ok ? iy : 0) * SZ;
   const int iz = 64 * qz + lane;
   const int izc = iz < Z ? iz : Z - 1;
   const bool publish = j == R - 1 && g + 1 < a.G && row_ok;
-  const bool consume = j == 0 && g > 0 && row_ok;
   double* bnd_out = a.bnd + ((long)c * a.G + g) * SZ;
-  const double* bnd_in = a.bnd + ((long)c * a.G + (g > 0 ? g - 1 : 0)) * SZ;
-  const int steps = S + R - 1;
-  auto slot = [&](int sl) { return sl < 0 ? 0 : (sl >= S ? S - 1 : sl); };
 
   auto load = [&](int T0, Chunk<C>& ch) {
 #pragma unroll
@@ -351,29 +431,11 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       ch.f0[q] = FALL ? a.wf[k] : 0.0;
     }
   };
-  auto load_by = [&](int T0, Bound<C>& by) {
-#pragma unroll
-    for (int q = 0; q < C; ++q)
-      by.v[q] = consume ? __hip_atomic_load(bnd_in + (long)slot(T0 + q - j) * Z + izc,
-                                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                        : 0.0;
-  };
   double prev = 0.0;
-  bool broken = false;
-  // runs the C steps from T0 with coefficients ch and boundary values by; the boundary
-  // values of the next chunk (into nby) are requested once the first step is done, and
-  // ch is refilled with the coefficients of the chunk K chunks later after the last step
-  auto run = [&](int T0, Chunk<C>& ch, Bound<C>& by, Bound<C>& nby) {
-    // the first row of a block waits until the previous block's last row has all C steps
-    if (consume && !broken) {
-      by = wait_boundary<C>(by, bnd_in, T0, j, iz, izc, S, Z, X, a.err);
-      broken = !by.ok;
-      // settle the boundary values inside this branch (an empty register constraint, no
-      // instruction), so no load of them is still counted as pending where the
-      // wavefronts that do not consume join again
-#pragma unroll
-      for (int q = 0; q < C; ++q) asm volatile("" : "+v"(by.v[q]));
-    }
+  // runs the C steps from T0 with coefficients ch, then refills ch with the coefficients
+  // of the chunk K chunks later
+  auto run = [&](int T0, Chunk<C>& ch) {
+    const int kc = T0 / C;
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int T = T0 + q;
@@ -382,14 +444,16 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       const bool valid = row_ok && s >= 0 && s < S && iz < Z && ix >= 0 && ix < X;
       const double below = __shfl(prev, (lane + 63) & 63);
       const int pb = ((T - 1) & 1) * R * Z, cb = (T & 1) * R * Z;
+      // row iy-1: the previous step of row j-1, or for the first row the boundary value
       double xy_l = 0.0, xz_l = 0.0;
-      if (j > 0 && iz < Z) xy_l = lds[pb + (j - 1) * Z + iz];
+      if (iz < Z)
+        xy_l = j > 0 ? lds[pb + (j - 1) * Z + iz] : byL[((kc & 1) * C + q) * kBoundStride + iz];
       if (lane == 0 && qz > 0) xz_l = lds[pb + j * Z + iz - 1];
       double out = 0.0;
       if (valid) {
         double t = ch.t[q];
         if (iz > 0) t = t - ch.w0[q] * (lane > 0 ? below : xz_l);
-        if (iy > 0) t = t - ch.w1[q] * (j > 0 ? xy_l : by.v[q]);
+        if (iy > 0) t = t - ch.w1[q] * xy_l;
         if (ix > 0) t = t - ch.w2[q] * prev;
         double dg = ch.w3[q];
         if (FALL) dg = dg + ch.f0[q] * alpha;
@@ -402,19 +466,17 @@ __global__ __launch_bounds__(512) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       if (iz < Z) lds[cb + j * Z + iz] = out;
       prev = out;
       __syncthreads();
-      if (q == 0) load_by(T0 + C, nby);
     }
     load(T0 + K * C, ch);
   };
   Chunk<C> buf[K];
-  Bound<C> byb[2];
 #pragma unroll
   for (int b = 0; b < K; ++b) load(b * C, buf[b]);
-  load_by(0, byb[0]);
+  __syncthreads();  // the first chunk's boundary values are in LDS
   for (int T0 = 0; T0 < steps; T0 += K * C) {
 #pragma unroll
     for (int b = 0; b < K; ++b)
-      if (T0 + b * C < steps) run(T0 + b * C, buf[b], byb[b & 1], byb[(b + 1) & 1]);
+      if (T0 + b * C < steps) run(T0 + b * C, buf[b]);
   }
 }
 
@@ -523,7 +585,8 @@ static int tralloc(asam_tr_ctx* c, T** p, size_t count) {
 
 static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// Rows per workgroup: at most 8 wavefronts (M per row, M = 2 when Z > 64).
+// Rows per workgroup: at most 8 compute wavefronts (M per row, M = 2 when Z > 64), plus
+// the one that polls the previous block's boundary row.
 static int rows_per_wg(const asam_tr_ctx* c) {
   int R = c->tune.gs_rows;
   if (R < 1) R = 1;
@@ -740,16 +803,16 @@ static int launch_matvec(asam_tr_ctx* c, const double* x, double* y, int init) {
 
 template <int C, int K, bool FALL>
 static void lower_launch(asam_tr_ctx* c, const GsArgs& a, int R, int Q) {
-  const size_t lds = 2 * (size_t)R * c->d.Z * sizeof(double);
-  k_gs_lower<C, K, FALL><<<(unsigned)(c->nc * a.G), 64 * R * Q, lds, c->stream>>>(a, c->ops_ps,
-                                                                                 Q);
+  const size_t lds = (2 * (size_t)R * c->d.Z + 2 * (size_t)C * kBoundStride) * sizeof(double);
+  k_gs_lower<C, K, FALL><<<(unsigned)(c->nc * a.G), 64 * (R * Q + 1), lds, c->stream>>>(
+      a, c->ops_ps, Q);
 }
 
-// gs_chunk 2: chunks of 2 steps, 6 in registers; 4: chunks of 4 steps, 2 in registers
+// gs_chunk 2: chunks of 2 steps, 4 in registers; 4: chunks of 4 steps, 2 in registers
 template <bool FALL>
 static void lower_kind(asam_tr_ctx* c, const GsArgs& a, int R) {
   if (c->tune.gs_chunk >= 4) lower_launch<4, 2, FALL>(c, a, R, c->M);
-  else lower_launch<2, 6, FALL>(c, a, R, c->M);
+  else lower_launch<2, 4, FALL>(c, a, R, c->M);
 }
 
 template <int NC>
@@ -1023,7 +1086,8 @@ extern "C" int asam_tr_set_tuning(asam_tr_ctx* c, const asam_tr_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
   if (t->gs_rows < 1 || t->gs_rows > 8) return trfail(c, ASAM_E_INVALID, "gs_rows");
   if (t->gs_chunk != 2 && t->gs_chunk != 4) return trfail(c, ASAM_E_INVALID, "gs_chunk");
-  if (2 * (size_t)t->gs_rows * c->d.Z * sizeof(double) > 60 * 1024)
+  if ((2 * (size_t)t->gs_rows * c->d.Z + 2 * (size_t)t->gs_chunk * kBoundStride) *
+          sizeof(double) > 60 * 1024)
     return trfail(c, ASAM_E_INVALID, "gs_rows: LDS");
   c->tune = *t;
   return 0;
