@@ -342,6 +342,10 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
   const int S = d.S, Z = d.Z, X = d.X;
   const long SZ = (long)S * Z;
   const int steps = S + R - 1;
+  // every wavefront runs whole loop turns of K chunks: the steps past `steps` compute
+  // nothing (no cell is valid there), and no branch skips part of a turn, so the
+  // compiler's load-wait counts stay exact across the turn instead of draining
+  const int nchunks = (steps + K * C - 1) / (K * C) * K;
   double* byL = lds + 2 * R * Z;
   auto slot = [&](int sl) { return sl < 0 ? 0 : (sl >= S ? S - 1 : sl); };
 
@@ -390,16 +394,14 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
       settle(0, 0);
     }
     __syncthreads();
-    for (int T0 = 0; T0 < steps; T0 += C) {
-      const int kc = T0 / C;
+    for (int kc = 0; kc < nchunks; ++kc) {
+      const int T0 = kc * C;
 #pragma unroll
       for (int q = 0; q < C; ++q) {
-        const int T = T0 + q;
-        if (T >= steps) break;
         // the next chunk's values: requested after this chunk's first barrier, in LDS
         // before its last one
         if (active && q == 0 && T0 + C < steps) issue(T0 + C);
-        if (active && (q == C - 1 || T == steps - 1) && T0 + C < steps) settle(T0 + C, kc + 1);
+        if (active && q == C - 1 && T0 + C < steps) settle(T0 + C, kc + 1);
         __syncthreads();
       }
     }
@@ -439,7 +441,6 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
 #pragma unroll
     for (int q = 0; q < C; ++q) {
       const int T = T0 + q;
-      if (T >= steps) break;  // uniform across the workgroup
       const int s = T - j, ix = s - iz;
       const bool valid = row_ok && s >= 0 && s < S && iz < Z && ix >= 0 && ix < X;
       const double below = __shfl(prev, (lane + 63) & 63);
@@ -473,10 +474,9 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
 #pragma unroll
   for (int b = 0; b < K; ++b) load(b * C, buf[b]);
   __syncthreads();  // the first chunk's boundary values are in LDS
-  for (int T0 = 0; T0 < steps; T0 += K * C) {
+  for (int T0 = 0; T0 < nchunks * C; T0 += K * C) {
 #pragma unroll
-    for (int b = 0; b < K; ++b)
-      if (T0 + b * C < steps) run(T0 + b * C, buf[b]);
+    for (int b = 0; b < K; ++b) run(T0 + b * C, buf[b]);
   }
 }
 
