@@ -541,7 +541,7 @@ struct asam_tr_ctx {
   Dims d{};
   int nc = 0, iterg = 5, M = 1;
   Ops ops_mv{}, ops_ps{};  // per-component operator of MatVec and of PSolve
-  asam_tr_tuning tune{8, 2, 0, 0};
+  asam_tr_tuning tune{8, 2, {0, 0}};
   bool have_mom = false, have_pot = false, have_fall = false, have_lu = false;
   std::string err;
   std::vector<void*> allocs;
