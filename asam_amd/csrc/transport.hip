@@ -196,6 +196,8 @@ struct GsArgs {
   int* counter;        // nc tickets
   int* err;            // set when a wait gives up
   int G;
+  int spin_limit;      // polls of a boundary value before a block gives up
+  int hold;            // test hook: no block publishes its last row
 };
 
 // t = b - U x_old on every valid slot (sweeps after the first), all components of a cell
@@ -258,7 +260,7 @@ struct Pend {
 template <int C>
 __device__ __attribute__((noinline)) Pend<C> settle_boundary(Pend<C> p, const double* bnd_in,
                                                            int T0, int S, int Z, int X,
-                                                           int* err) {
+                                                           int limit, int* err) {
   const int lane = threadIdx.x & 63;
   int n = 0;
   p.ok = true;
@@ -275,7 +277,7 @@ __device__ __attribute__((noinline)) Pend<C> settle_boundary(Pend<C> p, const do
       }
     }
     if (!__any(miss)) return p;
-    if (++n > kSpinLimit) {
+    if (++n > limit) {
       __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       p.ok = false;
       return p;
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
         }
       }
       if (__any(miss) && !broken) {
-        p = settle_boundary<C>(p, bnd_in, T0, S, Z, X, a.err);
+        p = settle_boundary<C>(p, bnd_in, T0, S, Z, X, a.spin_limit, a.err);
         broken = !p.ok;
       }
 #pragma unroll
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(576) void k_gs_lower(GsArgs a, Ops ops, int Q) {
   const long rowbase = (long)(row_ok ? iy : 0) * SZ;
   const int iz = 64 * qz + lane;
   const int izc = iz < Z ? iz : Z - 1;
-  const bool publish = j == R - 1 && g + 1 < a.G && row_ok;
+  const bool publish = j == R - 1 && g + 1 < a.G && row_ok && !a.hold;
   double* bnd_out = a.bnd + ((long)c * a.G + g) * SZ;
 
   auto load = [&](int T0, Chunk<C>& ch) {
@@ -541,7 +543,8 @@ struct asam_tr_ctx {
   Dims d{};
   int nc = 0, iterg = 5, M = 1;
   Ops ops_mv{}, ops_ps{};  // per-component operator of MatVec and of PSolve
-  asam_tr_tuning tune{8, 2, {0, 0}};
+  asam_tr_tuning tune{8, 2, 0, 0};
+  int hold_boundary = 0;  // test hook (asam_tr_test_hold_boundary)
   bool have_mom = false, have_pot = false, have_fall = false, have_lu = false;
   std::string err;
   std::vector<void*> allocs;
@@ -855,6 +858,8 @@ static int launch_sweep(asam_tr_ctx* c, int sweep) {
   a.counter = c->sync;
   a.err = c->sync + c->sync_n - 1;
   a.G = G;
+  a.spin_limit = c->tune.gs_spin_limit > 0 ? c->tune.gs_spin_limit : kSpinLimit;
+  a.hold = c->hold_boundary;
   if (sweep > 0)
     if (int rc = launch_upart(c, a)) return rc;
   k_zero_int<<<blocks_for(c->nc, 256), 256, 0, c->stream>>>(c->sync, c->nc);
@@ -1086,10 +1091,17 @@ extern "C" int asam_tr_set_tuning(asam_tr_ctx* c, const asam_tr_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
   if (t->gs_rows < 1 || t->gs_rows > 8) return trfail(c, ASAM_E_INVALID, "gs_rows");
   if (t->gs_chunk != 2 && t->gs_chunk != 4) return trfail(c, ASAM_E_INVALID, "gs_chunk");
+  if (t->gs_spin_limit < 0) return trfail(c, ASAM_E_INVALID, "gs_spin_limit");
   if ((2 * (size_t)t->gs_rows * c->d.Z + 2 * (size_t)t->gs_chunk * kBoundStride) *
           sizeof(double) > 60 * 1024)
     return trfail(c, ASAM_E_INVALID, "gs_rows: LDS");
   c->tune = *t;
+  return 0;
+}
+
+extern "C" int asam_tr_test_hold_boundary(asam_tr_ctx* c, int hold) {
+  if (!c) return ASAM_E_INVALID;
+  c->hold_boundary = hold != 0;
   return 0;
 }
 

@@ -38,7 +38,8 @@ class TrInfo(C.Structure):
 
 
 class TrTuning(C.Structure):
-    _fields_ = [("gs_rows", C.c_int), ("gs_chunk", C.c_int), ("reserved", C.c_int * 2)]
+    _fields_ = [("gs_rows", C.c_int), ("gs_chunk", C.c_int), ("gs_spin_limit", C.c_int),
+                ("reserved", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
@@ -69,6 +70,7 @@ def load():
         "asam_tr_get_tuning": (i, [_vp, C.POINTER(TrTuning)]),
         "asam_tr_set_tuning": (i, [_vp, C.POINTER(TrTuning)]),
         "asam_tr_time_kernel": (i, [_vp, i, i, _dp]),
+        "asam_tr_test_hold_boundary": (i, [_vp, i]),
         "asam_tr_dev_alloc": (i, [_vp, C.POINTER(_vp), C.c_size_t]),
         "asam_tr_dev_free": (i, [_vp, _vp]),
         "asam_tr_dev_upload": (i, [_vp, _vp, _vp, C.c_size_t]),
@@ -244,6 +246,11 @@ class TransportSolver:
         for k, v in kw.items():
             setattr(t, k, int(v))
         self._check(load().asam_tr_set_tuning(self.h, C.byref(t)))
+
+    def test_hold_boundary(self, hold=True):
+        """Test hook: no block publishes its boundary row, so later blocks wait until the
+        gs_spin_limit and the sweep fails (asam_tr_test_hold_boundary)."""
+        self._check(load().asam_tr_test_hold_boundary(self.h, int(bool(hold))))
 
     def time_kernel(self, which, reps=10):
         """Device ms of one Gauss-Seidel sweep (0), one MatVec (1), one PSolve (2)."""

@@ -71,10 +71,12 @@ typedef struct {
 
 /* Launch shapes of the Gauss-Seidel sweep (results do not depend on them). */
 typedef struct {
-  int gs_rows;  /* y-rows per workgroup: 1..8 (one wavefront per row; two and at most
-                   4 rows when nz > 62) */
-  int gs_chunk; /* steps whose coefficients a wavefront loads in one batch: 2 or 4 */
-  int reserved[2];
+  int gs_rows;       /* y-rows per workgroup: 1..8 (one wavefront per row; two and at most
+                        4 rows when nz > 62) */
+  int gs_chunk;      /* steps whose coefficients a wavefront loads in one batch: 2 or 4 */
+  int gs_spin_limit; /* polls of a boundary value before a block stops waiting for it (the
+                        call then returns ASAM_E_HIP); 0: 4194304 */
+  int reserved;
 } asam_tr_tuning;
 
 /* Creates the context and its device workspace (no per-call allocation). */
@@ -124,6 +126,12 @@ int asam_tr_set_tuning(asam_tr_ctx *ctx, const asam_tr_tuning *t);
  * components, 1 = the operator (MatVec), 2 = one whole PSolve.  No counterpart in the
  * reference (measurement). */
 int asam_tr_time_kernel(asam_tr_ctx *ctx, int which, int reps, double *ms);
+
+/* Test hook, no counterpart in the reference: with hold != 0 no block publishes its last
+ * row, so every later block waits for its boundary values until gs_spin_limit polls and
+ * the sweep returns ASAM_E_HIP, every kernel still finishing normally.  For tests of the
+ * bounded wait (with a small gs_spin_limit); hold = 0 restores normal sweeps. */
+int asam_tr_test_hold_boundary(asam_tr_ctx *ctx, int hold);
 
 /* Device buffers on the context's device (for the *_device entry points). */
 int asam_tr_dev_alloc(asam_tr_ctx *ctx, void **ptr, size_t bytes);

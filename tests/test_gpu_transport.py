@@ -158,6 +158,23 @@ def test_device_entry_points(trm, orc):
         v.free()
 
 
+def test_bounded_wait_reports_error(trm, orc):
+    """A block whose boundary row never arrives stops waiting after gs_spin_limit polls:
+    the call returns ASAM_E_HIP instead of hanging, every kernel finishes, and the context
+    sweeps correctly again once the hold is lifted."""
+    from asam_amd import sound
+    tp = problem(20, 30, 9, ncomp=2)
+    s = trm.TransportSolver.from_problem(tp)
+    s.set_tuning(gs_rows=2, gs_chunk=2, gs_spin_limit=64)
+    x = W.normal(tp.ncomp * tp.m, 49).reshape(tp.ncomp, tp.m)
+    s.test_hold_boundary(True)
+    with pytest.raises(sound.AsamError, match="waited past its limit"):
+        s.PSolve(x)
+    s.test_hold_boundary(False)
+    s.set_tuning(gs_spin_limit=0)
+    assert np.array_equal(s.PSolve(x), orc.TransportOracle(tp).psolve(x))
+
+
 def test_wrap_coupling_rejected(trm):
     from asam_amd import sound
     tp = problem(6, 4, 3, ncomp=1)
