@@ -568,14 +568,11 @@ void line_inner(asam_ens_ctx* c, const float* src, float* p, const float* v, flo
   e.partial = c->partial; e.pstride = c->pstride; e.qslot = qslot;
   const size_t lds = (size_t)c->maxnz * 64 * NP * sizeof(float);
   const bool use = c->tune.line_lds && lds <= lds_budget();
-  // two windows are in registers at once; two of 8 2x2 levels exceed the 256 architected
-  // VGPRs, so 2x2 blocks run windows of 4
-  constexpr int P8 = NP == 2 ? 4 : 8;
   if (use) {
-    if (c->tune.line_window == 8) line_go<NP, MODE, 1, P8>(c, e, lds);
+    if (c->tune.line_window == 8) line_go<NP, MODE, 1, 8>(c, e, lds);
     else line_go<NP, MODE, 1, 4>(c, e, lds);
   } else {
-    if (c->tune.line_window == 8) line_go<NP, MODE, 0, P8>(c, e, 0);
+    if (c->tune.line_window == 8) line_go<NP, MODE, 0, 8>(c, e, 0);
     else line_go<NP, MODE, 0, 4>(c, e, 0);
   }
 }

@@ -34,7 +34,7 @@ struct Coef {
 struct Tuning {
   int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread
   int apply_wg = 4096;    // target workgroups per apply launch
-  int line_window = 0;    // levels per load window of the line solve (1, 2, 4, 8; 0: by type)
+  int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
   int line_lds = 1;       // stage forward-sweep results in LDS (else in the output vector)
   int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
   int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)

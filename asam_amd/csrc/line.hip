@@ -1,7 +1,6 @@
 // line.hip -- vertical-line preconditioner solve (gfx950): one thread per column, 64
-// columns per wavefront (coalesced along x at every level), the loads of the next window
-// of levels issued before the Thomas recurrence of the current one runs, forward-sweep
-// results staged in LDS.
+// columns per wavefront (coalesced along x at every level), loads issued a window of
+// levels ahead of the Thomas recurrence, forward-sweep results staged in LDS.
 #include "line_tile.h"
 
 namespace asam {
@@ -55,29 +54,6 @@ static void line_launch(hipStream_t s, const Geom& g, const KLine<CT, VT>& a, in
   }
 }
 
-// Largest window: two windows are in registers at once, and every variant stays within
-// the 256 architected VGPRs (no spill to the accumulation registers).
-template <int NP, typename CT>
-constexpr int line_pmax() {
-  return NP == 2 ? (sizeof(CT) == 8 ? 2 : 4) : 8;
-}
-
-template <int NP, int TR, int LDS, typename CT, typename VT>
-static void line_pick(hipStream_t s, const Geom& g, const KLine<CT, VT>& a, int mode,
-                      size_t lds, int P) {
-  constexpr int PMAX = line_pmax<NP, CT>();
-  if constexpr (PMAX >= 8) {
-    if (P >= 8) { line_launch<NP, TR, LDS, 8, CT, VT>(s, g, a, mode, lds); return; }
-  }
-  if constexpr (PMAX >= 4) {
-    if (P >= 4) { line_launch<NP, TR, LDS, 4, CT, VT>(s, g, a, mode, lds); return; }
-  }
-  if constexpr (PMAX >= 2) {
-    if (P >= 2) { line_launch<NP, TR, LDS, 2, CT, VT>(s, g, a, mode, lds); return; }
-  }
-  line_launch<NP, TR, LDS, 1, CT, VT>(s, g, a, mode, lds);
-}
-
 template <int NP, int TR, typename CT, typename VT>
 static void line_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const LineReq& r) {
   KLine<CT, VT> a;
@@ -87,10 +63,18 @@ static void line_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const LineRe
   a.qslot = r.qslot;
   const size_t lds = (size_t)g.nz * 64 * NP * sizeof(CT);
   const bool use_lds = r.use_lds && g.tune.line_lds && lds <= lds_budget();
-  // window: the tuning's, else the largest that fits the registers
-  const int P = g.tune.line_window > 0 ? g.tune.line_window : line_pmax<NP, CT>();
-  if (use_lds) line_pick<NP, TR, 1, CT, VT>(s, g, a, r.mode, lds, P);
-  else line_pick<NP, TR, 0, CT, VT>(s, g, a, r.mode, 0, P);
+  // default window: 8 levels, 4 for FP64 2x2 blocks (register budget)
+  const int P = g.tune.line_window > 0 ? g.tune.line_window
+                                       : ((sizeof(CT) == 8 && NP == 2) ? 4 : 8);
+  if (use_lds) {
+    if (P >= 8) line_launch<NP, TR, 1, 8, CT, VT>(s, g, a, r.mode, lds);
+    else if (P >= 4) line_launch<NP, TR, 1, 4, CT, VT>(s, g, a, r.mode, lds);
+    else line_launch<NP, TR, 1, 1, CT, VT>(s, g, a, r.mode, lds);
+  } else {
+    if (P >= 8) line_launch<NP, TR, 0, 8, CT, VT>(s, g, a, r.mode, 0);
+    else if (P >= 4) line_launch<NP, TR, 0, 4, CT, VT>(s, g, a, r.mode, 0);
+    else line_launch<NP, TR, 0, 1, CT, VT>(s, g, a, r.mode, 0);
+  }
 }
 
 template <int NP, int TR>

@@ -122,65 +122,50 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
       }
     };
     // --- forward sweep: y_k = t_k^-1 (in_k - l_k y_k-1)   (transposed: in_k - w_k-1^T y_k-1)
-    // A window holds the loads of P levels.  The sweeps are software-pipelined: the loads
-    // of the next window are issued before the recurrence of the current one runs, so
-    // two windows are in flight per wavefront.  Each branch below holds a load block and
-    // a compute block together, so no control-flow join separates a window's loads from
-    // their use (a join would make the compiler drain every outstanding load there).
-    struct FWin {
-      V r[P], p[P], v[P];
-      M l[P], t[P];
-    };
-    struct BWin {
-      V y[P];
-      M w[P], t[P];
-    };
     V yprev{0, 0};
-    // levels past the top of the column (a partial last window) load level nz-1
-    auto fload = [&](int kb, FWin& W, auto first_t) {
-      constexpr bool FIRST = decltype(first_t)::value;
+    auto fwd = [&](int kb, auto chk) {
+      constexpr bool CHK = decltype(chk)::value;
+      V r_[P], p_[P], v_[P];
+      M l_[P], t_[P];
 #pragma unroll
       for (int q = 0; q < P; ++q) {
-        const int k = kb + q < nz ? kb + q : nz - 1;
-        const long c = c0 + sz * k;
-        if constexpr (MODE == LINE_PLAIN) W.r[q] = ldv(SRC, c);
-        if constexpr (MODE == LINE_BICG_P) {
-          W.r[q] = ldv(SRC, c);
-          if constexpr (!FIRST) { W.p[q] = ldv((const VT*)PP, c); W.v[q] = ldv(VV, c); }
+        const int k = kb + q;
+        if (!CHK || k < nz) {
+          const long c = c0 + sz * k;
+          if constexpr (MODE == LINE_PLAIN) r_[q] = ldv(SRC, c);
+          if constexpr (MODE == LINE_BICG_P) {
+            r_[q] = ldv(SRC, c);
+            if (!first) { p_[q] = ldv((const VT*)PP, c); v_[q] = ldv(VV, c); }
+          }
+          if constexpr (MODE == LINE_BICG_S) { r_[q] = ldv((const VT*)PP, c); v_[q] = ldv(VV, c); }
+          const long f = f0 + fs * k;
+          if constexpr (TR == 0) { l_[q] = ldm(PL, f); t_[q] = ldm(PT, f); }
+          else l_[q] = (k > 0) ? ldm(PW, f - fs) : M{{0, 0, 0, 0}};
         }
-        if constexpr (MODE == LINE_BICG_S) { W.r[q] = ldv((const VT*)PP, c); W.v[q] = ldv(VV, c); }
-        const long f = f0 + fs * k;
-        if constexpr (TR == 0) { W.l[q] = ldm(PL, f); W.t[q] = ldm(PT, f); }
-        else W.l[q] = ldm(PW, f0 + fs * (k > 0 ? k - 1 : 0));  // unused at k = 0
       }
-    };
-    auto fcomp = [&](int kb, auto chk, const FWin& W, auto first_t) {
-      constexpr bool CHK = decltype(chk)::value;
-      constexpr bool FIRST = decltype(first_t)::value;
 #pragma unroll
       for (int q = 0; q < P; ++q) {
         const int k = kb + q;
         if (!CHK || k < nz) {
           const long c = c0 + sz * k;
           V in;
-          if constexpr (MODE == LINE_PLAIN) in = V{W.r[q].a * sc, W.r[q].b * sc};
+          if constexpr (MODE == LINE_PLAIN) in = V{r_[q].a * sc, r_[q].b * sc};
           else if constexpr (MODE == LINE_BICG_P) {
-            if constexpr (FIRST) in = W.r[q];
-            else
-              in = V{W.r[q].a + beta * (W.p[q].a - omega * W.v[q].a),
-                     W.r[q].b + beta * (W.p[q].b - omega * W.v[q].b)};
+            in = first ? r_[q]
+                       : V{r_[q].a + beta * (p_[q].a - omega * v_[q].a),
+                           r_[q].b + beta * (p_[q].b - omega * v_[q].b)};
             stv(PP, c, in);
           } else {
-            in = V{W.r[q].a - alpha * W.v[q].a, W.r[q].b - alpha * W.v[q].b};
+            in = V{r_[q].a - alpha * v_[q].a, r_[q].b - alpha * v_[q].b};
             stv(PP, c, in);
             acc[0] += (double)in.a * (double)in.a + (double)in.b * (double)in.b;
           }
           V y;
           if constexpr (TR == 0) {
-            const V ly = mul(W.l[q], yprev);
-            y = mul(W.t[q], V{in.a - ly.a, in.b - ly.b});
+            const V ly = mul(l_[q], yprev);
+            y = mul(t_[q], V{in.a - ly.a, in.b - ly.b});
           } else {
-            const V wy = k > 0 ? mulT(W.l[q], yprev) : V{0, 0};
+            const V wy = mulT(l_[q], yprev);
             y = V{in.a - wy.a, in.b - wy.b};
           }
           stage_put(k, c, y);
@@ -190,22 +175,24 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
     };
     // --- backward sweep: z_k = y_k - w_k z_k+1   (transposed: t_k^-T (y_k - l_k+1^T z_k+1))
     V znext{0, 0};
-    auto bload = [&](int kb, BWin& W) {
+    auto bwd = [&](int kb, auto chk) {
+      constexpr bool CHK = decltype(chk)::value;
+      V y_[P];
+      M w_[P], t_[P];
 #pragma unroll
       for (int q = 0; q < P; ++q) {
-        const int k = kb + q < nz ? kb + q : nz - 1;
-        const long c = c0 + sz * k;
-        W.y[q] = stage_get(k, c);
-        const long f = f0 + fs * k;
-        if constexpr (TR == 0) W.w[q] = ldm(PW, f);
-        else {
-          W.w[q] = ldm(PL, k + 1 < nz ? f + fs : f);  // unused at k = nz - 1
-          W.t[q] = ldm(PT, f);
+        const int k = kb + q;
+        if (!CHK || k < nz) {
+          const long c = c0 + sz * k;
+          y_[q] = stage_get(k, c);
+          const long f = f0 + fs * k;
+          if constexpr (TR == 0) w_[q] = ldm(PW, f);
+          else {
+            w_[q] = (k + 1 < nz) ? ldm(PL, f + fs) : M{{0, 0, 0, 0}};
+            t_[q] = ldm(PT, f);
+          }
         }
       }
-    };
-    auto bcomp = [&](int kb, auto chk, const BWin& W) {
-      constexpr bool CHK = decltype(chk)::value;
 #pragma unroll
       for (int q = P - 1; q >= 0; --q) {
         const int k = kb + q;
@@ -213,73 +200,24 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
           const long c = c0 + sz * k;
           V z;
           if constexpr (TR == 0) {
-            const V wz = mul(W.w[q], znext);
-            z = V{W.y[q].a - wz.a, W.y[q].b - wz.b};
+            const V wz = mul(w_[q], znext);
+            z = V{y_[q].a - wz.a, y_[q].b - wz.b};
           } else {
-            const V lz = k + 1 < nz ? mulT(W.w[q], znext) : V{0, 0};
-            z = mulT(W.t[q], V{W.y[q].a - lz.a, W.y[q].b - lz.b});
+            const V lz = mulT(w_[q], znext);
+            z = mulT(t_[q], V{y_[q].a - lz.a, y_[q].b - lz.b});
           }
           stv(OUT, c, V{-z.a, -z.b});
           znext = z;
         }
       }
     };
-    using Full = std::false_type;
-    using Chk = std::true_type;
-    const int nwin = (nz + P - 1) / P;  // the last window may be partial
-    auto sweeps = [&](auto first_t) {
-      FWin X, Y;
-      fload(0, X, first_t);
-      for (int w = 0;; ++w) {  // X holds window w
-        if (w + 1 < nwin) {
-          fload((w + 1) * P, Y, first_t);
-          fcomp(w * P, Full{}, X, first_t);
-        } else {
-          fcomp(w * P, Chk{}, X, first_t);
-          break;
-        }
-        ++w;  // Y holds window w
-        if (w + 1 < nwin) {
-          fload((w + 1) * P, X, first_t);
-          fcomp(w * P, Full{}, Y, first_t);
-        } else {
-          fcomp(w * P, Chk{}, Y, first_t);
-          break;
-        }
-      }
-    };
-    if constexpr (MODE == LINE_BICG_P) {
-      if (first) sweeps(std::true_type{}); else sweeps(std::false_type{});
-    } else {
-      sweeps(std::false_type{});
+    const int nfull = nz / P;
+    for (int w = 0; w < nfull; ++w) fwd(w * P, std::false_type{});
+    if (nfull * P < nz) {
+      fwd(nfull * P, std::true_type{});
+      bwd(nfull * P, std::true_type{});
     }
-    {
-      BWin X, Y;
-      bload((nwin - 1) * P, X);
-      if (nwin > 1) {
-        bload((nwin - 2) * P, Y);
-        bcomp((nwin - 1) * P, Chk{}, X);
-        for (int w = nwin - 2;; --w) {  // Y holds window w
-          if (w > 0) {
-            bload((w - 1) * P, X);
-            bcomp(w * P, Full{}, Y);
-          } else {
-            bcomp(0, Full{}, Y);
-            break;
-          }
-          --w;  // X holds window w
-          if (w > 0) {
-            bload((w - 1) * P, Y);
-            bcomp(w * P, Full{}, X);
-          } else {
-            bcomp(0, Full{}, X);
-            break;
-          }
-        }
-      } else {
-        bcomp(0, Chk{}, X);
-      }
-    }
+    for (int w = nfull - 1; w >= 0; --w) bwd(w * P, std::false_type{});
   }
 }
 

@@ -133,8 +133,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   if (!c || !t) return ASAM_E_INVALID;
   if (t->apply_variant < 0 || t->apply_variant > 1) return fail(c, ASAM_E_INVALID, "apply_variant");
   if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
-  if (t->line_window != 0 && t->line_window != 1 && t->line_window != 2 && t->line_window != 4 &&
-      t->line_window != 8)
+  if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
     return fail(c, ASAM_E_INVALID, "line_window");
   Tuning nt;
   nt.apply_variant = t->apply_variant;

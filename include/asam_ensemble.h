@@ -34,7 +34,7 @@ typedef struct {
   int apply_bx;      /* stencil-apply tile width along x: 64, 128 or 256 (256 threads) */
   int apply_zsplit;  /* tiles per column along z (1..8): each member's nz split evenly */
   int apply_pf;      /* workgroups per CU of the persistent apply grid; 0: one per tile */
-  int line_window;   /* levels per load window of the line solve: 4 or 8 (2x2 blocks: 8 runs as 4) */
+  int line_window;   /* levels per load window of the line solve: 4 or 8 */
   int line_lds;      /* stage the forward sweep in LDS (1) or in the output vector (0) */
   int line_pf;       /* workgroups per CU of the persistent line grid; 0: one per tile */
   int blas_chunk;    /* vector elements per tile of the update kernels: 1024..65536 */

@@ -91,7 +91,7 @@ typedef struct {
 typedef struct {
   int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread */
   int apply_wg;      /* target workgroups per stencil-apply launch (256..16384) */
-  int line_window;   /* levels per load window of the line solve: 1, 2, 4 or 8 (0: default) */
+  int line_window;   /* levels per load window of the line solve: 1, 4 or 8 */
   int line_lds;      /* 1: stage the forward sweep in LDS, 0: in the output vector */
   int fused_tail;    /* reserved, must be 0 (reductions end in a separate finalize launch) */
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */

@@ -67,18 +67,3 @@ def test_no_cpu_fallback_in_product(sound_mod):
             m.load(os.path.join(ROOT, "does_not_exist.so"))
         finally:
             m._lib = saved
-
-
-def test_build_rejects_agpr_spills():
-    """The build's resource-usage check names every kernel that spills into AGPRs."""
-    from asam_amd.build import agpr_kernels
-    remarks = "\n".join([
-        "f.hip:1:1: remark: Function Name: _Z2k1v [-Rpass-analysis=kernel-resource-usage]",
-        "f.hip:1:1: remark:     VGPRs: 256 [-Rpass-analysis=kernel-resource-usage]",
-        "f.hip:1:1: remark:     AGPRs: 24 [-Rpass-analysis=kernel-resource-usage]",
-        "f.hip:9:1: remark: Function Name: _Z2k2v [-Rpass-analysis=kernel-resource-usage]",
-        "f.hip:9:1: remark:     VGPRs: 110 [-Rpass-analysis=kernel-resource-usage]",
-        "f.hip:9:1: remark:     AGPRs: 0 [-Rpass-analysis=kernel-resource-usage]",
-    ])
-    assert agpr_kernels(remarks) == [("_Z2k1v", 24)]
-    assert agpr_kernels("") == []

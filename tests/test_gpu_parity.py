@@ -173,30 +173,28 @@ def test_zero_guess_residual_form(np1, method, precision, oracle_mod, sound_mod)
         assert ia.iterations == ib.iterations and ia.status == ib.status == 0
 
 
-@pytest.mark.parametrize("precision", ["FP64", "MIXED"])
 @pytest.mark.parametrize("np1", [1, 2])
-def test_line_windows_agree(np1, precision, oracle_mod, sound_mod):
-    """Every load window of the line solve runs the same arithmetic one level at a time,
-    so the solves agree to round-off of the compiler's FMA contraction (and match the
-    oracle in FP64), for every window and with and without LDS staging of the sweep."""
+def test_line_windows_agree(np1, oracle_mod, sound_mod):
+    """The FP64 solve matches the oracle for every load window of the line solve, with and
+    without LDS staging of the sweep, and the windows agree with each other to round-off
+    (a wrong preconditioner still converges, but to a solution that differs at the
+    tolerance level, about 1e-9 here)."""
     p = W.config(1, np1=np1, scale=0.5)
     xo, mo, _, _ = oracle_mod.Oracle(p).bicgstab(p.b, 300, 1e-8)
     s = sound_mod.SoundSolver.from_problem(p)
-    s.set_control(precision=precision)
     ref = None
-    for win in (1, 2, 4, 8):
+    for win in (1, 4, 8):
         for lds in (0, 1):
             s.set_tuning(line_window=win, line_lds=lds)
             x, mg, tg, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
             if ref is None:
                 ref = x.copy()
             d, e = rel(x, ref), rel(x, xo)
-            print(f"np1={np1} {precision} window {win} lds {lds}: vs window 1 {d:.2e}, "
-                  f"vs oracle {e:.2e}, iterations {info.iterations}")
+            print(f"np1={np1} window {win} lds {lds}: vs window 1 {d:.2e}, vs oracle {e:.2e}, "
+                  f"MaxIter {mg} (oracle {mo})")
             assert info.status == 0
             assert d < 1e-13
-            if precision == "FP64":
-                assert e < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
+            assert e < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
 
 
 def test_zero_rhs_breakdown_semantics(oracle_mod, sound_mod):
