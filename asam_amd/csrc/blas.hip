@@ -148,6 +148,72 @@ void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs
                        use_vscale, st);
 }
 
+// ------------------------------------------------------- FGMRESV classical Gram-Schmidt
+// FGMRESV (Iter_Mod.f90:201-288) takes every projection H(k,i) = v_k . w of an Arnoldi
+// step from the same w (:238-241) before removing any (:242-244), so the projections need
+// no pass between them: up to 4 come from one read of w.  The basis is stored unscaled,
+// v_k = vs[k] * vscale[k], as in the modified Gram-Schmidt path.
+__global__ __launch_bounds__(256) void k_cgs_dots(long n, const double* __restrict__ w, ZList vl,
+                                                  int k0, int nq, const KState* st,
+                                                  double* partial) {
+  if (st->cycle_done) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0}, sk[4];
+  const double* v[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    sk[q] = q < nq ? st->vscale[k0 + q] : 0.0;
+    v[q] = q < nq ? (const double*)vl.z[k0 + q] : w;
+  }
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
+    const double wv = w[e];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (q < nq) acc[q] += (v[q][e] * sk[q]) * wv;
+  }
+  block_reduce_store_b<4>(acc, partial, blockIdx.x);
+}
+
+// w -= H(k,i) v_k for k = 0..i in that order (the DAXPYs of Iter_Mod.f90:242-244, each
+// rounded as a separate update), then |w|^2.
+__global__ __launch_bounds__(256) void k_cgs_update(long n, double* __restrict__ w, ZList vl,
+                                                    int i, const KState* st, double* partial) {
+  if (st->cycle_done) return;
+  double mh[kMaxM], sk[kMaxM];
+#pragma unroll
+  for (int k = 0; k < kMaxM; ++k) {
+    mh[k] = k <= i ? -st->H[k * kMaxM + i] : 0.0;
+    sk[k] = k <= i ? st->vscale[k] : 0.0;
+  }
+  double acc[1] = {0.0};
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
+    double wv = w[e];
+#pragma unroll
+    for (int k = 0; k < kMaxM; ++k)
+      if (k <= i) wv = wv + mh[k] * (((const double*)vl.z[k])[e] * sk[k]);
+    w[e] = wv;
+    acc[0] += wv * wv;
+  }
+  block_reduce_store_b<1>(acc, partial, blockIdx.x);
+}
+
+void launch_cgs_dots(hipStream_t s, long n, const double* w, double* const* vs, int k0, int nq,
+                     const KState* st, double* partial) {
+  ZList vl;  // vs holds at least k0 + nq vectors
+  for (int q = 0; q < kMaxM; ++q) vl.z[q] = q < k0 + nq ? vs[q] : nullptr;
+  hipLaunchKernelGGL(k_cgs_dots, dim3(blas_blocks(n)), dim3(256), 0, s, n, w, vl, k0, nq, st,
+                     partial);
+}
+
+void launch_cgs_update(hipStream_t s, long n, double* w, double* const* vs, int i,
+                       const KState* st, double* partial) {
+  ZList vl;  // vs holds at least i + 1 vectors
+  for (int q = 0; q < kMaxM; ++q) vl.z[q] = q <= i ? vs[q] : nullptr;
+  hipLaunchKernelGGL(k_cgs_update, dim3(blas_blocks(n)), dim3(256), 0, s, n, w, vl, i, st,
+                     partial);
+}
+
 // ------------------------------------------------------------- generic BLAS-1
 __global__ __launch_bounds__(256) void k_dot(long n, const double* x, const double* y,
                                              double* partial) {

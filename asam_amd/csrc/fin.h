@@ -104,6 +104,11 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       if (st->cycle_done) break;
       st->H[k * kMaxM + i] = (nq == 2) ? S[1] * st->vscale[0] : S[0];
       break;
+    case FIN_GM_HV:
+      // classical Gram-Schmidt: H[k + q][i] for the nq projections of one pass
+      if (st->cycle_done) break;
+      for (int q = 0; q < nq; ++q) st->H[(k + q) * kMaxM + i] = S[q];
+      break;
     case FIN_GM_LAST: {
       if (st->cycle_done) break;
       double* H = st->H;

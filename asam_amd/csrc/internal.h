@@ -168,6 +168,13 @@ void launch_mgs(hipStream_t s, long n, int vt, void* w, const void* vprev, const
 // x += sum_{q<nR} y_q * Z_q  (Z_q = z vectors, FGMRES) with optional scale vscale (GMRES).
 void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs, int m,
                          int use_vscale, const KState* st);
+// FGMRESV classical Gram-Schmidt of one Arnoldi step (Iter_Mod.f90:238-248): dots of the
+// new vector w with up to 4 basis vectors k0.. (partial slots 0..nq-1), and the removal of
+// all i+1 projections from w with |w|^2 (slot 0).  Basis vector k is vs[k] * vscale[k].
+void launch_cgs_dots(hipStream_t s, long n, const double* w, double* const* vs, int k0, int nq,
+                     const KState* st, double* partial);
+void launch_cgs_update(hipStream_t s, long n, double* w, double* const* vs, int i,
+                       const KState* st, double* partial);
 void launch_dot(hipStream_t s, long n, const double* x, const double* y, double* partial);
 void launch_axpby(hipStream_t s, long n, double a, const double* x, double b, double* y);
 void launch_fill(hipStream_t s, long n, int vt, void* x, double v);
@@ -187,6 +194,7 @@ enum FinKind {
   FIN_BICG_END,         // rnorm = |r|; exit tests; rho, beta for the next iteration
   FIN_GM_INIT,          // beta = |v0|; restart / exit bookkeeping
   FIN_GM_H,             // H[k][i] = sum   (k <= i)
+  FIN_GM_HV,            // H[k+q][i] = sum q, q < nq (classical Gram-Schmidt pass)
   FIN_GM_LAST,          // H[i+1][i] = |w|, Givens, convergence test
   FIN_DOT,              // store sum[0] to host-visible double (out)
   FIN_MIXED_OUTER,      // FP64 refinement residual -> next FP32 inner solve or exit

@@ -251,7 +251,7 @@ extern "C" int asam_sound_destroy(asam_sound_ctx* c) {
 
 extern "C" int asam_sound_set_control(asam_sound_ctx* c, const asam_sound_control* ctl) {
   if (!c || !ctl) return ASAM_E_INVALID;
-  if (ctl->method < 0 || ctl->method > 2) return fail(c, ASAM_E_INVALID, "method");
+  if (ctl->method < 0 || ctl->method > 3) return fail(c, ASAM_E_INVALID, "method");
   if (ctl->step_gmres < 1 || ctl->step_gmres > kMaxM) return fail(c, ASAM_E_INVALID, "step_gmres");
   if (ctl->precision < 0 || ctl->precision > 1) return fail(c, ASAM_E_INVALID, "precision");
   if (ctl->precision == ASAM_PREC_MIXED && ctl->method != ASAM_METHOD_BICGSTAB)
@@ -461,6 +461,21 @@ struct Enq {
     c->counts[2]++;
     fin(FIN_BICG_END, 2, blas_blocks(c->n));
   }
+  // classical Gram-Schmidt (FP64): nq projections of w on basis vectors k0.., then the
+  // removal of all i + 1 projections with |w|^2
+  void cgs_dots(double* w, double* const* vs, int k0, int nq, int i) {
+    const double nv = (double)c->n;
+    timed(4, (nq + 1) * nv * 8,
+          [&] { launch_cgs_dots(s, c->n, w, vs, k0, nq, c->st, c->partial); });
+    c->counts[2]++;
+    fin(FIN_GM_HV, nq, blas_blocks(c->n), k0, i);
+  }
+  void cgs_update(double* w, double* const* vs, int i) {
+    const double nv = (double)c->n;
+    timed(4, (i + 3) * nv * 8, [&] { launch_cgs_update(s, c->n, w, vs, i, c->st, c->partial); });
+    c->counts[2]++;
+    fin(FIN_GM_LAST, 1, blas_blocks(c->n), 0, i);
+  }
   void mgs(int vt, void* w, const void* vprev, const void* vk, int k, int i, int chk = 0) {
     const double ws = vt ? 4 : 8, nv = (double)c->n;
     double vecs = (k > 0 ? 3 : 1) + (k == i + 1 ? 0 : 1);
@@ -609,9 +624,11 @@ static int ensure_gmres(asam_sound_ctx* c, int m) {
   return 0;
 }
 
-// FGMRES(m) (Iter_Mod.f90:121-199) or left-preconditioned GMRES(m) (:32-119), FP64.
-static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, const double* b,
-                            int maxcycles, double tol, asam_sound_info* info) {
+// FGMRES(m) (Iter_Mod.f90:121-199), FGMRESV(m) (:201-288: FGMRES with classical
+// Gram-Schmidt) or left-preconditioned GMRES(m) (:32-119), FP64.
+static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classical, double* x,
+                            const double* b, int maxcycles, double tol,
+                            asam_sound_info* info) {
   const int m = c->ctl.step_gmres;
   int rc = ensure_gmres(c, m);
   if (rc) return rc;
@@ -657,6 +674,13 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, double* x, 
                &st->cycle_done);
         e.mgs(0, c->gv[i + 1], nullptr, c->gv[0], 0, i);
         e.fin(FIN_GM_H, 1, blas_blocks(n), 0, i);
+      }
+      if (classical) {
+        // H(0,i) came with the apply; the other projections use the same w
+        for (int k0 = 1; k0 <= i; k0 += 4)
+          e.cgs_dots(c->gv[i + 1], c->gv.data(), k0, i + 1 - k0 < 4 ? i + 1 - k0 : 4, i);
+        e.cgs_update(c->gv[i + 1], c->gv.data(), i);
+        continue;
       }
       for (int k = 1; k <= i; ++k) {
         e.mgs(0, c->gv[i + 1], c->gv[k - 1], c->gv[k], k, i);
@@ -789,7 +813,8 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
   } else if (c->ctl.method == ASAM_METHOD_BICGSTAB) {
     rc = solve_bicg_fp64(c, tr, x, b, maxiter, tol, &tmp);
   } else {
-    rc = solve_gmres_fp64(c, tr, c->ctl.method == ASAM_METHOD_FGMRES, x, b, *maxiter, *tol,
+    rc = solve_gmres_fp64(c, tr, c->ctl.method != ASAM_METHOD_GMRES,
+                          c->ctl.method == ASAM_METHOD_FGMRESV, x, b, *maxiter, *tol,
                           &tmp);
   }
   if (rc) return rc;

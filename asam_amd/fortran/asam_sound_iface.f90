@@ -12,8 +12,10 @@ module asam_sound_iface
   integer(c_int), parameter :: ASAM_METHOD_BICGSTAB = 0
   integer(c_int), parameter :: ASAM_METHOD_FGMRES = 1
   integer(c_int), parameter :: ASAM_METHOD_GMRES = 2
+  integer(c_int), parameter :: ASAM_METHOD_FGMRESV = 3
   integer(c_int), parameter :: ASAM_PREC_FP64 = 0
   integer(c_int), parameter :: ASAM_PREC_MIXED = 1
+  integer(c_int), parameter :: ASAM_E_INVALID = -1
 
   type, bind(c) :: asam_sound_grid
     integer(c_int) :: nx, ny, nz

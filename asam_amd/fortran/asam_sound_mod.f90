@@ -39,6 +39,7 @@ contains
     type(asam_sound_grid) :: g
     type(asam_sound_control) :: ctl
     integer(c_int) :: rc
+    logical :: missing
     if (c_associated(ctx)) call AsamSoundFinalize()
     g%nx = nx; g%ny = ny; g%nz = nz; g%np1 = np1
     g%grad_full = merge(1, 0, GradFull)
@@ -46,18 +47,27 @@ contains
     rc = asam_sound_create(ctx, g)
     call check(rc, 'asam_sound_create')
     rc = asam_sound_default_control(ctl)
+    ! Iter_Mod.f90:14-29; BICGSTAB is this library's addition.  QMR, MINRES and CGR are
+    ! not provided: AsamStatus reports ASAM_E_INVALID and the default method stays.
+    missing = .false.
     select case (trim(MethSound))
     case ('FGMRES')
       ctl%method = ASAM_METHOD_FGMRES
-    case ('GMRES')
-      ctl%method = ASAM_METHOD_GMRES
-    case default
+    case ('FGMRESV')
+      ctl%method = ASAM_METHOD_FGMRESV
+    case ('BICGSTAB')
       ctl%method = ASAM_METHOD_BICGSTAB
+    case ('QMR', 'MINRES', 'CGR')
+      write(0, '(a,a)') 'AsamSoundInit: MethSound not provided: ', trim(MethSound)
+      missing = .true.
+    case default
+      ctl%method = ASAM_METHOD_GMRES
     end select
     ctl%step_gmres = StepGMRES
     if (Mixed) ctl%precision = ASAM_PREC_MIXED
     rc = asam_sound_set_control(ctx, ctl)
     call check(rc, 'asam_sound_set_control')
+    if (missing) AsamStatus = ASAM_E_INVALID
   end subroutine AsamSoundInit
 
   !> Operator set-up from geometry and thermodynamic state (JacAccGrav).

@@ -32,6 +32,7 @@ program drv_solve_sound
   meth = 'BICGSTAB'
   if (method == 1) meth = 'FGMRES'
   if (method == 2) meth = 'GMRES'
+  if (method == 3) meth = 'FGMRESV'
   call AsamSoundInit(nx, ny, nz, np1, gradfull == 1, beta0dt, grav, 1.0d0, meth, 10, &
                      mixed == 1)
   call AsamJacAccGrav(FU, FV, FW, VolC, dx, dy, dz, rho, rth, T)

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libasam_sound.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "asam_sound.h")
 
-METHODS = {"BICGSTAB": 0, "FGMRES": 1, "GMRES": 2}
+METHODS = {"BICGSTAB": 0, "FGMRES": 1, "GMRES": 2, "FGMRESV": 3}
 PRECISIONS = {"FP64": 0, "MIXED": 1}
 
 _dp = C.POINTER(C.c_double)

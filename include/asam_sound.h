@@ -40,6 +40,7 @@ extern "C" {
 #define ASAM_METHOD_BICGSTAB 0 /* SpMatrix4CellPar_Mod.f90:245-348 sequence */
 #define ASAM_METHOD_FGMRES 1   /* Iter_Mod.f90:121-199 (reference default) */
 #define ASAM_METHOD_GMRES 2    /* Iter_Mod.f90:32-119 (left preconditioned) */
+#define ASAM_METHOD_FGMRESV 3  /* Iter_Mod.f90:201-288 (FGMRES, classical Gram-Schmidt) */
 
 /* Arithmetic of the solve. */
 #define ASAM_PREC_FP64 0  /* plain FP64 drop-in */

@@ -20,7 +20,8 @@
  *   - line solve:    Solve_1D (TriDiagT_Mod.f90:135-155; TriDiagTB_Mod.f90:143-163 with
  *                    CG/Matrix_Mod.f90 Mult_22/Mult_2/Div_22/Div_2 :34-94)
  *   - Krylov:        BiCGStab SpMatrix4CellPar_Mod.f90:245-348;
- *                    FGMRES / GMRES SOLU/Iter_Mod.f90:121-199 / :32-119
+ *                    FGMRES / FGMRESV / GMRES SOLU/Iter_Mod.f90:121-199 / :201-288 /
+ *                    :32-119
  *
  * The preconditioner is the vertical-line block-Jacobi: every z-column's
  * tri-diagonal (or 2x2 block tri-diagonal) line matrix of the assembled
@@ -603,8 +604,14 @@ int orc_bicgstab(const orc_op *A, double *x, const double *b, int *maxiter, doub
 
 /* ----------------------------------------------------------------- FGMRES */
 #define MAXM 64
-int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
-               int tr, double *rel_res) {
+/* FGMRES (Iter_Mod.f90:121-199) orthogonalises each new Arnoldi vector by modified
+ * Gram-Schmidt: H(k,i) is taken after the projections on v_1..v_k-1 are removed.
+ * FGMRESV (Iter_Mod.f90:201-288) is the same iteration with classical Gram-Schmidt: every
+ * H(k,i) is the product of v_k with the same copy w of the new vector (:238-241), then all
+ * projections are removed in order k = 1..i (:242-244).  For one closed block its
+ * Exchange(w) calls are copies and its 'Rho' print is output only. */
+static int fgmres_run(const orc_op *A, double *x, const double *b, int maxiter, double tol,
+                      int m, int tr, double *rel_res, int classical) {
   const long n = orc_n(A);
   if (m > MAXM) m = MAXM;
   double **v = (double **)malloc(sizeof(double *) * (m + 1));
@@ -626,9 +633,14 @@ int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double 
       pre(A, z[i], v[i], tr);
       op(A, v[i + 1], z[i], tr);
       ++total;
-      for (int k = 0; k <= i; ++k) {
-        H[k][i] = orc_dot(n, v[k], v[i + 1]);
-        axpy(n, -H[k][i], v[k], v[i + 1]);
+      if (classical) {
+        for (int k = 0; k <= i; ++k) H[k][i] = orc_dot(n, v[k], v[i + 1]);
+        for (int k = 0; k <= i; ++k) axpy(n, -H[k][i], v[k], v[i + 1]);
+      } else {
+        for (int k = 0; k <= i; ++k) {
+          H[k][i] = orc_dot(n, v[k], v[i + 1]);
+          axpy(n, -H[k][i], v[k], v[i + 1]);
+        }
       }
       H[i + 1][i] = sqrt(orc_dot(n, v[i + 1], v[i + 1]));
       scal(n, 1.0 / H[i + 1][i], v[i + 1]);
@@ -661,6 +673,16 @@ int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double 
   for (int i = 0; i < m; ++i) free(z[i]);
   free(v); free(z);
   return total;
+}
+
+int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
+               int tr, double *rel_res) {
+  return fgmres_run(A, x, b, maxiter, tol, m, tr, rel_res, 0);
+}
+
+int orc_fgmresv(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
+                int tr, double *rel_res) {
+  return fgmres_run(A, x, b, maxiter, tol, m, tr, rel_res, 1);
 }
 
 /* ------------------------------------------------------------------ GMRES */

@@ -64,6 +64,9 @@ int orc_bicgstab(const orc_op *A, double *x, const double *b, int *maxiter, doub
 /* FGMRES(m) (Iter_Mod.f90:121-199): maxiter = restart cycles. Returns total Arnoldi steps. */
 int orc_fgmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
                int transpose, double *rel_res);
+/* FGMRESV(m) (Iter_Mod.f90:201-288): FGMRES with classical Gram-Schmidt. */
+int orc_fgmresv(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
+                int transpose, double *rel_res);
 /* GMRES(m), left preconditioned (Iter_Mod.f90:32-119). Returns total Arnoldi steps. */
 int orc_gmres(const orc_op *A, double *x, const double *b, int maxiter, double tol, int m,
               int transpose, double *rel_res);

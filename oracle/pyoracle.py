@@ -53,7 +53,7 @@ def lib():
         L.orc_bicgstab.restype = C.c_int
         L.orc_bicgstab.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(C.c_int),
                                    C.POINTER(C.c_double), C.c_int]
-        for f in ("orc_fgmres", "orc_gmres"):
+        for f in ("orc_fgmres", "orc_fgmresv", "orc_gmres"):
             getattr(L, f).restype = C.c_int
             getattr(L, f).argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int,
                                       C.c_int, _dp]
@@ -188,6 +188,13 @@ class Oracle:
         rr = C.c_double(0.0)
         it = lib().orc_fgmres(self._h, _p(x), _p(_c(b)), maxiter, tol, m, int(transpose),
                               C.byref(rr))
+        return x, it, rr.value
+
+    def fgmresv(self, b, maxiter=10, tol=1e-8, m=10, transpose=False, x0=None):
+        x = np.zeros(self.n) if x0 is None else _c(x0).copy()
+        rr = C.c_double(0.0)
+        it = lib().orc_fgmresv(self._h, _p(x), _p(_c(b)), maxiter, tol, m, int(transpose),
+                               C.byref(rr))
         return x, it, rr.value
 
     def gmres(self, b, maxiter=10, tol=1e-8, m=10, transpose=False, x0=None):

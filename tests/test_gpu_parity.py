@@ -97,12 +97,12 @@ def test_bicgstab_fp64_parity(name, mk, oracle_mod, sound_mod):
     assert np.linalg.norm(r) <= 1.0001e-8 * np.linalg.norm(p.b)
 
 
-@pytest.mark.parametrize("method", ["FGMRES", "GMRES"])
+@pytest.mark.parametrize("method", ["FGMRES", "FGMRESV", "GMRES"])
 @pytest.mark.parametrize("np1", [1, 2])
 def test_gmres_fp64_parity(method, np1, oracle_mod, sound_mod):
     p = W.config(4, np1=np1, scale=0.5)
     orc = oracle_mod.Oracle(p)
-    f = orc.fgmres if method == "FGMRES" else orc.gmres
+    f = {"FGMRES": orc.fgmres, "FGMRESV": orc.fgmresv, "GMRES": orc.gmres}[method]
     xo, ito, rro = f(p.b, 20, 1e-8, 10)
     s = sound_mod.SoundSolver.from_problem(p)
     s.set_control(method=method, step_gmres=10)
@@ -113,6 +113,12 @@ def test_gmres_fp64_parity(method, np1, oracle_mod, sound_mod):
     assert abs(info.iterations - ito) <= 1
     assert e < SOL_TOL
     assert mi == 20  # FGMRES/GMRES leave MaxIter unchanged
+    if method == "FGMRESV":  # the adjoint solve runs the same iteration with -A^T, -M^-T
+        xo, ito, _ = f(p.b, 20, 1e-8, 10, transpose=True)
+        x, mi, tl, info = s.SolveSound_T(np.zeros(p.n), p.b, 20, 1e-8)
+        print(f"{method} np1={np1} transposed: oracle steps {ito}, gpu {info.iterations}, "
+              f"diff {rel(x, xo):.3e}")
+        assert abs(info.iterations - ito) <= 1 and rel(x, xo) < SOL_TOL
 
 
 @pytest.mark.parametrize("np1", [1, 2])
@@ -147,7 +153,8 @@ def test_mixed_precision_reaches_1e8(np1, oracle_mod, sound_mod):
 
 
 @pytest.mark.parametrize("method,precision", [("BICGSTAB", "FP64"), ("BICGSTAB", "MIXED"),
-                                              ("FGMRES", "FP64"), ("GMRES", "FP64")])
+                                              ("FGMRES", "FP64"), ("FGMRESV", "FP64"),
+                                              ("GMRES", "FP64")])
 @pytest.mark.parametrize("np1", [1, 2])
 def test_zero_guess_residual_form(np1, method, precision, oracle_mod, sound_mod):
     """The initial residual of an all-zero guess comes from the x = 0 form (b outside the

@@ -217,10 +217,27 @@ def test_krylov_solvers_match_direct_solve(np1, oracle_mod):
     assert st == 0 and rel(x, ref) < 1e-10
     x, it, rr = o.fgmres(p.b, 30, 1e-13, 10)
     assert rel(x, ref) < 1e-10
+    x, it, rr = o.fgmresv(p.b, 30, 1e-13, 10)
+    assert rel(x, ref) < 1e-10
     x, it, rr = o.gmres(p.b, 30, 1e-13, 10)
     assert rel(x, ref) < 1e-10
     x, mi, tl, st = o.bicgstab(p.b, 200, 1e-13, transpose=True)
     assert rel(x, refT) < 1e-10
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_fgmresv_is_fgmres_with_classical_gram_schmidt(np1, oracle_mod):
+    """Classical and modified Gram-Schmidt span the same Krylov basis, so FGMRESV takes
+    FGMRES's steps to round-off (same step count, same solution to 1e-12), yet the two
+    orthogonalisations round differently, so the results are not the same bits."""
+    p = tiny(np1)
+    o = oracle_mod.Oracle(p)
+    for tr in (False, True):
+        xm, im, rm = o.fgmres(p.b, 5, 1e-9, 10, transpose=tr)
+        xc, ic, rc = o.fgmresv(p.b, 5, 1e-9, 10, transpose=tr)
+        assert ic == im and rc <= 1e-9
+        assert np.linalg.norm(xc - xm) <= 1e-12 * np.linalg.norm(xm)
+        assert not np.array_equal(xc, xm)
 
 
 def test_bicgstab_reference_return_conventions(oracle_mod):
