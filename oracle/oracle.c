@@ -1,8 +1,8 @@
 /*
  * oracle.c -- CPU restatement (FP64) of ASAM's implicit sound stage solve.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the CPU
- * baseline timed beside it.  The product path never links or calls this file.
+ * TEST INFRASTRUCTURE ONLY: the reference answers the HIP library is tested against,
+ * and the CPU baseline timed beside it.  The product path never links or calls this file.
  *
  * What is restated (reference paths relative to SRC/):
  *   - face weights:  MULTI/JacAccGrav_Mod.f90 IniJacAccGrav :286-302 (DUU),

@@ -1,8 +1,8 @@
 /*
  * oracle.h -- CPU restatement (FP64) of ASAM's sound stage solve.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
- * path and the CPU baseline timed beside it.  Nothing in the product
+ * TEST INFRASTRUCTURE ONLY.  This library gives the reference answers the HIP product
+ * path is tested against, and the CPU baseline timed beside it.  Nothing in the product
  * (asam_amd/, include/) links or calls it.
  *
  * Every function cites the reference routine it restates (paths relative to

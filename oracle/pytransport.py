@@ -1,6 +1,6 @@
 """ctypes binding of the CPU transport oracle (transport.c).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke`` as the checker
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke`` for its reference answer
 and by bench.py's cpu_baseline leg.  The product never imports it.
 """
 from __future__ import annotations

@@ -2,8 +2,9 @@
  * transport.h -- CPU restatement (FP64) of ASAM's transport stage solve: BiCGStab over
  * the SpDiag transport Jacobian with the Gauss-Seidel preconditioner.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP transport path and the CPU
- * baseline timed beside it; nothing in the product (asam_amd/, include/) links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  The reference answers the HIP transport path is tested
+ * against, and the CPU baseline timed beside it; nothing in the product (asam_amd/,
+ * include/) links or calls it.
  *
  * Layout (as in the reference, paths relative to its SRC/ directory):
  *   - one block with a one-cell halo: m = n = (nx+2)(ny+2)(nz+2) entries, x fastest
