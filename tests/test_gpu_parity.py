@@ -121,6 +121,29 @@ def test_gmres_fp64_parity(method, np1, oracle_mod, sound_mod):
         assert abs(info.iterations - ito) <= 1 and rel(x, xo) < SOL_TOL
 
 
+@pytest.mark.parametrize("m", [1, 3, 32])
+@pytest.mark.parametrize("method", ["FGMRES", "FGMRESV"])
+def test_gmres_restart_lengths(method, m, oracle_mod, sound_mod):
+    """Restart lengths from 1 (a single projection per cycle) to the largest the library
+    takes.  m = 32 runs one whole cycle with Tol = 0: every projection of 32 Arnoldi steps
+    (for classical Gram-Schmidt, 8 groups of 4 at the last step), to round-off."""
+    p = W.config(4, np1=2, scale=0.5)
+    orc = oracle_mod.Oracle(p)
+    f = orc.fgmresv if method == "FGMRESV" else orc.fgmres
+    cycles, tol = {1: (60, 1e-8), 3: (20, 1e-8), 32: (1, 0.0)}[m]
+    xo, ito, rro = f(p.b, cycles, tol, m)
+    s = sound_mod.SoundSolver.from_problem(p)
+    s.set_control(method=method, step_gmres=m)
+    x, mi, tl, info = s.SolveSound(np.zeros(p.n), p.b, cycles, tol)
+    e = rel(x, xo)
+    if m == 32:
+        assert ito == 32 and info.iterations == 32
+    print(f"{method} m={m}: oracle steps {ito} rel {rro:.2e}; gpu steps {info.iterations} "
+          f"cycles {info.restarts} rel {info.rel_residual:.2e}; diff {e:.3e}")
+    assert abs(info.iterations - ito) <= 1
+    assert e < SOL_TOL
+
+
 @pytest.mark.parametrize("np1", [1, 2])
 def test_transposed_solve_parity(np1, oracle_mod, sound_mod):
     p = W.config(4, np1=np1, scale=0.5)
