@@ -153,30 +153,32 @@ void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs
 // step from the same w (:238-241) before removing any (:242-244), so the projections need
 // no pass between them: up to 4 come from one read of w.  The basis is stored unscaled,
 // v_k = vs[k] * vscale[k], as in the modified Gram-Schmidt path.
-__global__ __launch_bounds__(256) void k_cgs_dots(long n, const double* __restrict__ w, ZList vl,
+template <typename VT>
+__global__ __launch_bounds__(256) void k_cgs_dots(long n, const VT* __restrict__ w, ZList vl,
                                                   int k0, int nq, const KState* st,
                                                   double* partial) {
   if (st->cycle_done) return;
   double acc[4] = {0.0, 0.0, 0.0, 0.0}, sk[4];
-  const double* v[4];
+  const VT* v[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     sk[q] = q < nq ? st->vscale[k0 + q] : 0.0;
-    v[q] = q < nq ? (const double*)vl.z[k0 + q] : w;
+    v[q] = q < nq ? (const VT*)vl.z[k0 + q] : w;
   }
   const long stride = (long)gridDim.x * blockDim.x;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    const double wv = w[e];
+    const double wv = (double)w[e];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-      if (q < nq) acc[q] += (v[q][e] * sk[q]) * wv;
+      if (q < nq) acc[q] += ((double)v[q][e] * sk[q]) * wv;
   }
   block_reduce_store_b<4>(acc, partial, blockIdx.x);
 }
 
 // w -= H(k,i) v_k for k = 0..i in that order (the DAXPYs of Iter_Mod.f90:242-244, each
-// rounded as a separate update), then |w|^2.
-__global__ __launch_bounds__(256) void k_cgs_update(long n, double* __restrict__ w, ZList vl,
+// rounded to the vector type as a separate update), then |w|^2.
+template <typename VT>
+__global__ __launch_bounds__(256) void k_cgs_update(long n, VT* __restrict__ w, ZList vl,
                                                     int i, const KState* st, double* partial) {
   if (st->cycle_done) return;
   double mh[kMaxM], sk[kMaxM];
@@ -188,30 +190,38 @@ __global__ __launch_bounds__(256) void k_cgs_update(long n, double* __restrict__
   double acc[1] = {0.0};
   const long stride = (long)gridDim.x * blockDim.x;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    double wv = w[e];
+    double wv = (double)w[e];
 #pragma unroll
     for (int k = 0; k < kMaxM; ++k)
-      if (k <= i) wv = wv + mh[k] * (((const double*)vl.z[k])[e] * sk[k]);
-    w[e] = wv;
+      if (k <= i) wv = (double)(VT)(wv + mh[k] * ((double)((const VT*)vl.z[k])[e] * sk[k]));
+    w[e] = (VT)wv;
     acc[0] += wv * wv;
   }
   block_reduce_store_b<1>(acc, partial, blockIdx.x);
 }
 
-void launch_cgs_dots(hipStream_t s, long n, const double* w, double* const* vs, int k0, int nq,
-                     const KState* st, double* partial) {
+void launch_cgs_dots(hipStream_t s, long n, int vt, const void* w, void* const* vs, int k0,
+                     int nq, const KState* st, double* partial) {
   ZList vl;  // vs holds at least k0 + nq vectors
   for (int q = 0; q < kMaxM; ++q) vl.z[q] = q < k0 + nq ? vs[q] : nullptr;
-  hipLaunchKernelGGL(k_cgs_dots, dim3(blas_blocks(n)), dim3(256), 0, s, n, w, vl, k0, nq, st,
-                     partial);
+  if (vt == 0)
+    hipLaunchKernelGGL(k_cgs_dots<double>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
+                       (const double*)w, vl, k0, nq, st, partial);
+  else
+    hipLaunchKernelGGL(k_cgs_dots<float>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
+                       (const float*)w, vl, k0, nq, st, partial);
 }
 
-void launch_cgs_update(hipStream_t s, long n, double* w, double* const* vs, int i,
+void launch_cgs_update(hipStream_t s, long n, int vt, void* w, void* const* vs, int i,
                        const KState* st, double* partial) {
   ZList vl;  // vs holds at least i + 1 vectors
   for (int q = 0; q < kMaxM; ++q) vl.z[q] = q <= i ? vs[q] : nullptr;
-  hipLaunchKernelGGL(k_cgs_update, dim3(blas_blocks(n)), dim3(256), 0, s, n, w, vl, i, st,
-                     partial);
+  if (vt == 0)
+    hipLaunchKernelGGL(k_cgs_update<double>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
+                       (double*)w, vl, i, st, partial);
+  else
+    hipLaunchKernelGGL(k_cgs_update<float>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
+                       (float*)w, vl, i, st, partial);
 }
 
 // ------------------------------------------------------------- generic BLAS-1

@@ -171,6 +171,36 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       st->outer += 1;
       break;
     }
+    case FIN_MIXED_GM: {
+      // FP64 residual of the refinement step; sets up one FP32 (F)GMRES cycle on A d = r
+      // from d = 0 (the restart of the reference's cycle loop is the next refinement step)
+      if (st->outer_done) break;
+      if (st->outer > 0) {  // Arnoldi steps of the previous step's cycle
+        st->inner_total += st->total;
+        if (st->outer <= 16) st->hist[st->outer - 1] = st->total;
+      }
+      st->xpar = st->outer & 1;
+      const double r = sqrt(S[0]);
+      if (st->outer == 0) st->outer_r0 = r;
+      st->outer_r = r;
+      if (r <= st->outer_tol * st->outer_r0 || r == 0.0) {
+        st->outer_done = 1; st->gstatus = 0; st->done = 1; st->pad = 1; st->cycle_done = 1;
+        break;
+      }
+      double need = 0.5 * st->outer_tol * st->outer_r0 / r;
+      double itol = st->inner_tol;
+      if (need > itol) itol = need;
+      if (itol > 0.5) itol = 0.5;
+      st->gtol = itol;
+      st->total = 0; st->cycles = 0;
+      st->grho0 = r; st->gbeta = r; st->grho = r;
+      st->vscale[0] = 1.0 / r;
+      st->bhat[0] = r;
+      st->nR = st->m;
+      st->done = 0; st->pad = 0; st->cycle_done = 0;
+      st->outer += 1;
+      break;
+    }
     default:
       break;
   }

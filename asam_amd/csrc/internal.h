@@ -171,9 +171,9 @@ void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs
 // FGMRESV classical Gram-Schmidt of one Arnoldi step (Iter_Mod.f90:238-248): dots of the
 // new vector w with up to 4 basis vectors k0.. (partial slots 0..nq-1), and the removal of
 // all i+1 projections from w with |w|^2 (slot 0).  Basis vector k is vs[k] * vscale[k].
-void launch_cgs_dots(hipStream_t s, long n, const double* w, double* const* vs, int k0, int nq,
-                     const KState* st, double* partial);
-void launch_cgs_update(hipStream_t s, long n, double* w, double* const* vs, int i,
+void launch_cgs_dots(hipStream_t s, long n, int vt, const void* w, void* const* vs, int k0,
+                     int nq, const KState* st, double* partial);
+void launch_cgs_update(hipStream_t s, long n, int vt, void* w, void* const* vs, int i,
                        const KState* st, double* partial);
 void launch_dot(hipStream_t s, long n, const double* x, const double* y, double* partial);
 void launch_axpby(hipStream_t s, long n, double a, const double* x, double b, double* y);
@@ -198,6 +198,7 @@ enum FinKind {
   FIN_GM_LAST,          // H[i+1][i] = |w|, Givens, convergence test
   FIN_DOT,              // store sum[0] to host-visible double (out)
   FIN_MIXED_OUTER,      // FP64 refinement residual -> next FP32 inner solve or exit
+  FIN_MIXED_GM,         // FP64 refinement residual -> one FP32 (F)GMRES cycle or exit
 };
 // Quantities q < 2 were written by nblk workgroups, q >= 2 by nblk_hi workgroups.
 void launch_finalize(hipStream_t s, int kind, int nq, int nblk, const double* partial,

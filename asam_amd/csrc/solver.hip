@@ -45,6 +45,8 @@ struct asam_sound_ctx {
   // (F)GMRES basis
   int gm_m = 0;
   std::vector<double*> gv, gz;
+  int gm32_m = 0;  // FP32 (F)GMRES basis of the mixed solve
+  std::vector<float*> gv32, gz32;
   double* gtmp = nullptr;
   std::vector<void*> allocs;
   int counts[8] = {0};
@@ -254,8 +256,9 @@ extern "C" int asam_sound_set_control(asam_sound_ctx* c, const asam_sound_contro
   if (ctl->method < 0 || ctl->method > 3) return fail(c, ASAM_E_INVALID, "method");
   if (ctl->step_gmres < 1 || ctl->step_gmres > kMaxM) return fail(c, ASAM_E_INVALID, "step_gmres");
   if (ctl->precision < 0 || ctl->precision > 1) return fail(c, ASAM_E_INVALID, "precision");
-  if (ctl->precision == ASAM_PREC_MIXED && ctl->method != ASAM_METHOD_BICGSTAB)
-    return fail(c, ASAM_E_INVALID, "mixed precision runs the BiCGStab inner solver");
+  if (ctl->precision == ASAM_PREC_MIXED && ctl->method == ASAM_METHOD_GMRES)
+    return fail(c, ASAM_E_INVALID,
+                "mixed precision runs a BiCGStab, FGMRES or FGMRESV inner solver");
   c->ctl = *ctl;
   return 0;
 }
@@ -461,18 +464,19 @@ struct Enq {
     c->counts[2]++;
     fin(FIN_BICG_END, 2, blas_blocks(c->n));
   }
-  // classical Gram-Schmidt (FP64): nq projections of w on basis vectors k0.., then the
-  // removal of all i + 1 projections with |w|^2
-  void cgs_dots(double* w, double* const* vs, int k0, int nq, int i) {
-    const double nv = (double)c->n;
-    timed(4, (nq + 1) * nv * 8,
-          [&] { launch_cgs_dots(s, c->n, w, vs, k0, nq, c->st, c->partial); });
+  // classical Gram-Schmidt: nq projections of w on basis vectors k0.., then the removal
+  // of all i + 1 projections with |w|^2
+  void cgs_dots(int vt, void* w, void* const* vs, int k0, int nq, int i) {
+    const double nv = (double)c->n, ws = vt ? 4 : 8;
+    timed(4 + vt, (nq + 1) * nv * ws,
+          [&] { launch_cgs_dots(s, c->n, vt, w, vs, k0, nq, c->st, c->partial); });
     c->counts[2]++;
     fin(FIN_GM_HV, nq, blas_blocks(c->n), k0, i);
   }
-  void cgs_update(double* w, double* const* vs, int i) {
-    const double nv = (double)c->n;
-    timed(4, (i + 3) * nv * 8, [&] { launch_cgs_update(s, c->n, w, vs, i, c->st, c->partial); });
+  void cgs_update(int vt, void* w, void* const* vs, int i) {
+    const double nv = (double)c->n, ws = vt ? 4 : 8;
+    timed(4 + vt, (i + 3) * nv * ws,
+          [&] { launch_cgs_update(s, c->n, vt, w, vs, i, c->st, c->partial); });
     c->counts[2]++;
     fin(FIN_GM_LAST, 1, blas_blocks(c->n), 0, i);
   }
@@ -624,6 +628,20 @@ static int ensure_gmres(asam_sound_ctx* c, int m) {
   return 0;
 }
 
+static int ensure_gmres32(asam_sound_ctx* c, int m) {
+  if (c->gm32_m >= m) return 0;
+  for (float* q : c->gv32) dfree(c, q);
+  for (float* q : c->gz32) dfree(c, q);
+  c->gv32.assign(m + 1, nullptr);
+  c->gz32.assign(m, nullptr);
+  int rc = 0;
+  for (int i = 0; i <= m; ++i) rc |= dalloc(c, &c->gv32[i], c->n);
+  for (int i = 0; i < m; ++i) rc |= dalloc(c, &c->gz32[i], c->n);
+  if (rc) return ASAM_E_NOMEM;
+  c->gm32_m = m;
+  return 0;
+}
+
 // FGMRES(m) (Iter_Mod.f90:121-199), FGMRESV(m) (:201-288: FGMRES with classical
 // Gram-Schmidt) or left-preconditioned GMRES(m) (:32-119), FP64.
 static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classical, double* x,
@@ -677,9 +695,10 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classic
       }
       if (classical) {
         // H(0,i) came with the apply; the other projections use the same w
+        void* const* vs = (void* const*)c->gv.data();
         for (int k0 = 1; k0 <= i; k0 += 4)
-          e.cgs_dots(c->gv[i + 1], c->gv.data(), k0, i + 1 - k0 < 4 ? i + 1 - k0 : 4, i);
-        e.cgs_update(c->gv[i + 1], c->gv.data(), i);
+          e.cgs_dots(0, c->gv[i + 1], vs, k0, i + 1 - k0 < 4 ? i + 1 - k0 : 4, i);
+        e.cgs_update(0, c->gv[i + 1], vs, i);
         continue;
       }
       for (int k = 1; k <= i; ++k) {
@@ -796,6 +815,87 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   return 0;
 }
 
+// FP32-inner FGMRES / FGMRESV with FP64 iterative refinement.  Each refinement step is
+// the FP64 residual r = b - Op(x) (FP64 coefficients; written as the FP32 first basis
+// vector) and one FP32 cycle of at most m Arnoldi steps on A d = r from d = 0, to the inner
+// tolerance; then x += d is fused into the next step's residual.  The reference's restart
+// (a new cycle from the current iterate, Iter_Mod.f90:136-139) is the next refinement step,
+// so every cycle starts from an FP64 residual.
+static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, const double* b,
+                          double tol, asam_sound_info* info) {
+  const int m = c->ctl.step_gmres;
+  int rc = ensure_gmres32(c, m);
+  if (rc) return rc;
+  Enq e(c, tr);
+  KState* hs = c->st_host;
+  memset(hs, 0, sizeof(KState));
+  hs->outer_tol = tol;
+  hs->inner_tol = c->ctl.inner_tol;
+  hs->m = m;
+  hs->maxcycles = 1;
+  hs->xz = 1;
+  rc = upload_state(c);
+  if (rc) return rc;
+  KState* st = c->st;
+  const Geom& g = c->g;
+  const long n = c->n;
+  double* const bufs[2] = {x, (x == c->xalt) ? c->xd : c->xalt};
+  float* const v0 = c->gv32[0];
+  auto outer_step = [&](int k) {
+    if (k == 0)
+      e.resid_init(1, bufs[0], v0, b, c->rt32, DOT_YY, FIN_MIXED_GM);
+    else
+      e.apply(0, 0, 1, 0, bufs[(k - 1) & 1], v0, b, c->rt32, nullptr, DOT_YY, &st->outer_done,
+              c->d32, bufs[k & 1], 0, FIN_MIXED_GM, 1);
+  };
+  outer_step(0);
+  rc = sync_state(c);
+  if (rc) return rc;
+  int k = 0;
+  while (!hs->outer_done && k < c->ctl.max_outer) {
+    e.timed(5, 4.0 * n, [&] { launch_fill(e.s, n, 1, c->d32, 0.0); });
+    for (int i = 0; i < m; ++i) {
+      e.line(1, LINE_PLAIN, c->gv32[i], nullptr, nullptr, c->gz32[i], &st->vscale[i],
+             &st->cycle_done);
+      e.apply(1, 1, 1, 1, c->gz32[i], c->gv32[i + 1], nullptr, nullptr, v0, DOT_YU,
+              &st->cycle_done);
+      e.fin(FIN_GM_H, 2, apply_blocks(g), 0, i);
+      if (classical) {
+        void* const* vs = (void* const*)c->gv32.data();
+        for (int k0 = 1; k0 <= i; k0 += 4)
+          e.cgs_dots(1, c->gv32[i + 1], vs, k0, i + 1 - k0 < 4 ? i + 1 - k0 : 4, i);
+        e.cgs_update(1, c->gv32[i + 1], vs, i);
+        continue;
+      }
+      for (int q = 1; q <= i; ++q) {
+        e.mgs(1, c->gv32[i + 1], c->gv32[q - 1], c->gv32[q], q, i);
+        e.fin(FIN_GM_H, 1, blas_blocks(n), q, i);
+      }
+      e.mgs(1, c->gv32[i + 1], c->gv32[i], nullptr, i + 1, i);
+      e.fin(FIN_GM_LAST, 1, blas_blocks(n), 0, i);
+    }
+    launch_gm_backsolve(e.s, st);
+    launch_gmres_update(e.s, n, 1, c->d32, (void* const*)c->gz32.data(), m, 0, st);
+    c->counts[2]++;
+    ++k;
+    outer_step(k);
+    rc = sync_state(c);
+    if (rc) return rc;
+  }
+  const bool converged = hs->outer_done != 0;
+  double* xfin = bufs[hs->xpar & 1];
+  if (xfin != x) HIPCHK(c, hipMemcpyAsync(x, xfin, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (info) {
+    info->iterations = hs->inner_total;
+    info->restarts = hs->outer;
+    info->status = converged ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
+    info->abs_residual = hs->outer_r;
+    info->rel_residual = hs->outer_r0 > 0 ? hs->outer_r / hs->outer_r0 : 0.0;
+  }
+  return 0;
+}
+
 static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double* b, int* maxiter,
                              double* tol, asam_sound_info* info) {
   if (!c) return ASAM_E_INVALID;
@@ -808,7 +908,9 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
   memset(&tmp, 0, sizeof tmp);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int rc;
-  if (c->ctl.precision == ASAM_PREC_MIXED) {
+  if (c->ctl.precision == ASAM_PREC_MIXED && c->ctl.method != ASAM_METHOD_BICGSTAB) {
+    rc = solve_mixed_gm(c, tr, c->ctl.method == ASAM_METHOD_FGMRESV, x, b, *tol, &tmp);
+  } else if (c->ctl.precision == ASAM_PREC_MIXED) {
     rc = solve_mixed(c, tr, x, b, *tol, &tmp);
   } else if (c->ctl.method == ASAM_METHOD_BICGSTAB) {
     rc = solve_bicg_fp64(c, tr, x, b, maxiter, tol, &tmp);

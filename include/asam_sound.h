@@ -45,7 +45,7 @@ extern "C" {
 /* Arithmetic of the solve. */
 #define ASAM_PREC_FP64 0  /* plain FP64 drop-in */
 #define ASAM_PREC_MIXED 1 /* FP32 operator/preconditioner/inner Krylov; FP64 residual,
-                             dots and refinement */
+                             dots and refinement (BiCGStab, FGMRES or FGMRESV) */
 
 /* Solve status (asam_sound_info.status). */
 #define ASAM_CONVERGED 0

@@ -157,22 +157,30 @@ def test_transposed_solve_parity(np1, oracle_mod, sound_mod):
     assert e < SOL_TOL
 
 
+@pytest.mark.parametrize("method", ["BICGSTAB", "FGMRES", "FGMRESV"])
 @pytest.mark.parametrize("np1", [1, 2])
-def test_mixed_precision_reaches_1e8(np1, oracle_mod, sound_mod):
+def test_mixed_precision_reaches_1e8(np1, method, oracle_mod, sound_mod):
+    """FP32 inner solves (BiCGStab, or one FGMRES / FGMRESV cycle per refinement step)
+    with FP64 refinement reach the 1e-8 relative residual of the FP64 system (recomputed
+    by the oracle's operator), forward and transposed; the error against the oracle's
+    FP64 solve is stated."""
     p = W.config(4, np1=np1, scale=0.5)
     orc = oracle_mod.Oracle(p)
-    xo, mo, to, so = orc.bicgstab(p.b, 300, 1e-12)
     s = sound_mod.SoundSolver.from_problem(p)
-    s.set_control(precision="MIXED")
-    x, mi, tl, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
-    r = p.b - orc.matvec1(x)
-    rr = np.linalg.norm(r) / np.linalg.norm(p.b)
-    err = rel(x, xo)
-    print(f"mixed np1={np1}: inner its {info.iterations}, outer {info.restarts}, "
-          f"true rel residual {rr:.3e}, error vs FP64 oracle {err:.3e}")
-    assert info.status == 0
-    assert rr <= 1.0001e-8
-    assert err < 1e-6
+    s.set_control(method=method, precision="MIXED", step_gmres=10)
+    for tr in (False, True):
+        xo, mo, to, so = orc.bicgstab(p.b, 300, 1e-12, transpose=tr)
+        solve = s.SolveSound_T if tr else s.SolveSound
+        x, mi, tl, info = solve(np.zeros(p.n), p.b, 300, 1e-8)
+        r = p.b - (orc.matvec1_T(x) if tr else orc.matvec1(x))
+        rr = np.linalg.norm(r) / np.linalg.norm(p.b)
+        err = rel(x, xo)
+        print(f"mixed {method} np1={np1} {'transposed' if tr else 'forward'}: inner its "
+              f"{info.iterations}, outer {info.restarts}, true rel residual {rr:.3e}, "
+              f"error vs FP64 oracle {err:.3e}")
+        assert info.status == 0
+        assert rr <= 1.0001e-8
+        assert err < 1e-6
 
 
 @pytest.mark.parametrize("method,precision", [("BICGSTAB", "FP64"), ("BICGSTAB", "MIXED"),
