@@ -107,6 +107,30 @@ def test_fortran_solve_sound_matches_oracle(np1, method, oracle_mod, tmp_path):
 
 @needs_flang
 @pytest.mark.gpu
+@pytest.mark.parametrize("method", [0, 1])
+def test_fortran_mixed_solve_reaches_1e8(method, oracle_mod, tmp_path):
+    """AsamSoundInit(..., Mixed=.TRUE.) with BiCGStab or the reference's default FGMRES:
+    the FP32-inner / FP64-refined solve reaches the 1e-8 relative residual of the FP64
+    system, recomputed by the oracle's operator."""
+    drv = ensure_built()
+    p = W.config(4, np1=2, scale=0.5)
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, p, method=method, mixed=1, maxiter=200 if method == 0 else 20)
+    r = subprocess.run([drv, fin, fout], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    maxiter, tol, rel, its, x = read_output(fout, p.n)
+    orc = oracle_mod.Oracle(p)
+    rr = np.linalg.norm(p.b - orc.matvec1(x)) / np.linalg.norm(p.b)
+    xo, _, _, _ = orc.bicgstab(p.b, 300, 1e-12)
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"Fortran mixed method={method}: inner its {its}, rel residual {rr:.2e}, "
+          f"err vs FP64 oracle {err:.2e}")
+    assert rr <= 1.0001e-8 and err < 1e-6
+
+
+@needs_flang
+@pytest.mark.gpu
 @pytest.mark.parametrize("np1", [1, 2])
 def test_fortran_project_vel_face_matches_oracle(np1, oracle_mod, tmp_path):
     """Rhs -> SolveSound -> UpdateVelocity from Fortran, against the oracle's sequence."""
