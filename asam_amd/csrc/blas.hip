@@ -78,14 +78,41 @@ void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, 
 }
 
 // ------------------------------------------------------------- (F)GMRES MGS
+// Sum of the nb partial products of the previous pass, in the same fixed order in every
+// workgroup (so every workgroup holds the same bits); all threads must take part.
+__device__ double block_sum_partials(const double* p, int nb) {
+  __shared__ double sh[4];
+  __shared__ double total;
+  const int tid = threadIdx.x;
+  double v = 0.0;
+  for (int b = tid; b < nb; b += 256) v += p[b];
+  v = wave_sum(v);
+  if ((tid & 63) == 0) sh[tid >> 6] = v;
+  __syncthreads();
+  if (tid == 0) total = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return total;
+}
+
+// hsrc: null, or the partial products of the previous pass of the chain; then H(k-1, i) is
+// their sum, formed here (workgroup 0 records it) instead of by a finalize launch.
 template <typename VT>
 __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
                                              const VT* __restrict__ vprev,
                                              const VT* __restrict__ vk, int k, int i,
-                                             const KState* st, double* partial, int chk_done) {
+                                             KState* st, double* partial, int chk_done,
+                                             const double* hsrc, int hnb) {
   if (chk_done ? st->done : st->cycle_done) return;
   double h = 0.0, sp = 0.0, sk = 0.0;
-  if (k > 0) { h = st->H[(k - 1) * kMaxM + i]; sp = st->vscale[k - 1]; }
+  if (k > 0) {
+    if (hsrc) {
+      h = block_sum_partials(hsrc, hnb);
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->H[(k - 1) * kMaxM + i] = h;
+    } else {
+      h = st->H[(k - 1) * kMaxM + i];
+    }
+    sp = st->vscale[k - 1];
+  }
   const bool self = (k == i + 1);
   if (!self) sk = st->vscale[k];
   double acc[1] = {0.0};
@@ -104,14 +131,16 @@ __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
 }
 
 void launch_mgs(hipStream_t s, long n, int vt, void* w, const void* vprev, const void* vk, int k,
-                int i, const KState* st, double* partial, int chk_done) {
-  int nb = blas_blocks(n);
+                int i, KState* st, double* partial, int chk_done, const double* hsrc) {
+  const int nb = blas_blocks(n);
   if (vt == 0)
     hipLaunchKernelGGL(k_mgs<double>, dim3(nb), dim3(256), 0, s, n, (double*)w,
-                       (const double*)vprev, (const double*)vk, k, i, st, partial, chk_done);
+                       (const double*)vprev, (const double*)vk, k, i, st, partial, chk_done,
+                       hsrc, nb);
   else
     hipLaunchKernelGGL(k_mgs<float>, dim3(nb), dim3(256), 0, s, n, (float*)w,
-                       (const float*)vprev, (const float*)vk, k, i, st, partial, chk_done);
+                       (const float*)vprev, (const float*)vk, k, i, st, partial, chk_done,
+                       hsrc, nb);
 }
 
 struct ZList {

@@ -162,9 +162,12 @@ void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, 
                         void* r, const void* t, const void* rt, const KState* st,
                         double* partial);
 // (F)GMRES modified Gram-Schmidt step: w -= H[k-1][i]*vscale[k-1]*v_{k-1} (k>0); then
-// q0 = w.v_k (k<=i) or w.w (k==i+1).
+// q0 = w.v_k (k<=i) or w.w (k==i+1), into partial slot 0 of `partial`.  hsrc: null (H[k-1][i]
+// is in the state), or the previous step's partial products, whose sum is H[k-1][i]
+// (recorded in the state by workgroup 0).
 void launch_mgs(hipStream_t s, long n, int vt, void* w, const void* vprev, const void* vk,
-                int k, int i, const KState* st, double* partial, int chk_done = 0);
+                int k, int i, KState* st, double* partial, int chk_done = 0,
+                const double* hsrc = nullptr);
 // x += sum_{q<nR} y_q * Z_q  (Z_q = z vectors, FGMRES) with optional scale vscale (GMRES).
 void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs, int m,
                          int use_vscale, const KState* st);

@@ -447,12 +447,11 @@ struct Enq {
           [&] { launch_line(s, c->g, c->c64, c->c32, r); });
     c->counts[1]++;
   }
-  void fin(int kind, int nq, int nblk, int k = 0, int i = 0, int tag = 0, int nblk_hi = 0) {
+  void fin(int kind, int nq, int nblk, int k = 0, int i = 0, int tag = 0, int nblk_hi = 0,
+           const double* base = nullptr) {
+    const double* p = base ? base : c->partial;
     timed(6, 8.0 * nq * nblk,
-          [&] {
-            launch_finalize(s, kind, nq, nblk, c->partial, c->st, k, i, nullptr, nblk_hi);
-          },
-          tag);
+          [&] { launch_finalize(s, kind, nq, nblk, p, c->st, k, i, nullptr, nblk_hi); }, tag);
     c->counts[3]++;
   }
   void bicg_update(int vt, void* x, const void* ph, const void* rh, void* r, const void* t,
@@ -486,6 +485,27 @@ struct Enq {
     timed(4 + vt, vecs * nv * ws,
           [&] { launch_mgs(s, c->n, vt, w, vprev, vk, k, i, c->st, c->partial, chk); });
     c->counts[2]++;
+  }
+  // Steps k = 1..i+1 of the modified Gram-Schmidt chain of Arnoldi step i (w = vs[i+1]),
+  // H(0,i) already in the state.  Step k >= 2 sums the previous step's partial products
+  // itself (the same fixed order in every workgroup), so only the last step ends in a
+  // finalize; the steps alternate between partial slots 2 and 3.
+  void mgs_chain(int vt, void* const* vs, int i) {
+    const double ws = vt ? 4 : 8, nv = (double)c->n;
+    const double* prev = nullptr;
+    int slot = 2;
+    for (int k = 1; k <= i + 1; ++k) {
+      const bool self = k == i + 1;
+      double* out = c->partial + (long)slot * kRedBlocks;
+      timed(4 + vt, (self ? 3 : 4) * nv * ws, [&] {
+        launch_mgs(s, c->n, vt, vs[i + 1], vs[k - 1], self ? nullptr : vs[k], k, i, c->st, out,
+                   0, prev);
+      });
+      c->counts[2]++;
+      prev = out;
+      slot ^= 1;
+    }
+    fin(FIN_GM_LAST, 1, blas_blocks(c->n), 0, i, 0, 0, prev);
   }
 };
 
@@ -701,12 +721,7 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classic
         e.cgs_update(0, c->gv[i + 1], vs, i);
         continue;
       }
-      for (int k = 1; k <= i; ++k) {
-        e.mgs(0, c->gv[i + 1], c->gv[k - 1], c->gv[k], k, i);
-        e.fin(FIN_GM_H, 1, blas_blocks(n), k, i);
-      }
-      e.mgs(0, c->gv[i + 1], c->gv[i], nullptr, i + 1, i);
-      e.fin(FIN_GM_LAST, 1, blas_blocks(n), 0, i);
+      e.mgs_chain(0, (void* const*)c->gv.data(), i);
     }
     launch_gm_backsolve(e.s, st);
     if (flexible)
@@ -867,12 +882,7 @@ static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, c
         e.cgs_update(1, c->gv32[i + 1], vs, i);
         continue;
       }
-      for (int q = 1; q <= i; ++q) {
-        e.mgs(1, c->gv32[i + 1], c->gv32[q - 1], c->gv32[q], q, i);
-        e.fin(FIN_GM_H, 1, blas_blocks(n), q, i);
-      }
-      e.mgs(1, c->gv32[i + 1], c->gv32[i], nullptr, i + 1, i);
-      e.fin(FIN_GM_LAST, 1, blas_blocks(n), 0, i);
+      e.mgs_chain(1, (void* const*)c->gv32.data(), i);
     }
     launch_gm_backsolve(e.s, st);
     launch_gmres_update(e.s, n, 1, c->d32, (void* const*)c->gz32.data(), m, 0, st);
