@@ -355,7 +355,12 @@ __global__ __launch_bounds__(1024) void k_finalize(int kind, int nq, int nblk, i
   __shared__ KState sk;
   __shared__ double sh[4][16];
   static_assert(sizeof(KState) % 8 == 0, "KState is copied in 8-byte words");
-  constexpr int kWords = (int)(sizeof(KState) / 8);
+  static_assert(offsetof(KState, H) % 8 == 0, "the scalar part is copied in 8-byte words");
+  // the BiCGStab and refinement tails read and write only the scalars ahead of the
+  // (F)GMRES arrays (about 350 bytes of the state's 10 KiB)
+  const bool arrays = kind == FIN_GM_INIT || kind == FIN_GM_H || kind == FIN_GM_HV ||
+                      kind == FIN_GM_LAST || kind == FIN_MIXED_GM;
+  const int kWords = arrays ? (int)(sizeof(KState) / 8) : (int)(offsetof(KState, H) / 8);
   const int tid = threadIdx.x;
   for (int w = tid; w < kWords; w += 1024)
     reinterpret_cast<unsigned long long*>(&sk)[w] =

@@ -70,9 +70,7 @@ struct KState {
   double sum[4];
   int iter, maxiter, done, status;
   int half, result, first, pad;
-  // (F)GMRES
-  double H[(kMaxM + 1) * kMaxM];  // H[row*kMaxM + col]
-  double cs[kMaxM], sn[kMaxM], bhat[kMaxM + 1], y[kMaxM], vscale[kMaxM + 1];
+  // (F)GMRES scalars
   double gbeta, grho, grho0, gtol;
   int m, step, nR, cycle_done;
   int total, cycles, maxcycles, gstatus;
@@ -84,6 +82,9 @@ struct KState {
   int xnz, xz;          // initial guess has a nonzero entry / is all zero (k_any_nonzero)
   int pad4, pad5;
   int hist[16];         // inner iterations of refinement step q (q < 16)
+  // (F)GMRES arrays, last: the tails that do not use them copy only the part above
+  double H[(kMaxM + 1) * kMaxM];  // H[row*kMaxM + col]
+  double cs[kMaxM], sn[kMaxM], bhat[kMaxM + 1], y[kMaxM], vscale[kMaxM + 1];
 };
 
 // ------------------------------------------------------------------ set-up
