@@ -97,6 +97,25 @@ def timed_solves(s, xbuf, bbuf, steps, warmup, tol, profile):
     return (t1 - t0) / steps, infos, prof
 
 
+def fgmres_leg(s, xbuf, bbuf, steps, tol):
+    """FGMRES (the reference's default SolveSound method, Iter_Mod.f90:121-199) and FGMRESV
+    (:201-288), m = 10, in FP64 and FP32-inner / FP64-refined, on the same resident
+    system as the headline solve."""
+    out = {"workload": "same system and tolerance, restart length m = 10, x = 0"}
+    for method in ("FGMRES", "FGMRESV"):
+        row = {}
+        for prec in ("FP64", "MIXED"):
+            s.set_control(method=method, step_gmres=10, precision=prec)
+            t, infos, _ = timed_solves(s, xbuf, bbuf, steps, 1, tol, False)
+            row[prec.lower() + "_ms"] = round(t * 1e3, 3)
+            row[prec.lower() + "_steps"] = infos[-1]["iterations"]
+            row[prec.lower() + "_rel_residual"] = infos[-1]["rel_residual"]
+        row["speedup_mixed_vs_fp64"] = round(row["fp64_ms"] / row["mixed_ms"], 3)
+        out[method] = row
+    s.set_control(precision="MIXED")
+    return out
+
+
 def roofline_from_profile(prof, np1):
     # dominant kernel class = most device time inside the timed region
     best = max(prof.items(), key=lambda kv: kv[1]["ms"])
@@ -483,6 +502,7 @@ def main():
     diff_vs_fp64_gpu = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
     line_proj = projection_leg(s, p, max(2, a.steps // 4), a.tol)
+    fgm = fgmres_leg(s, xbuf, bbuf, max(2, a.steps // 4), a.tol)
 
     roof, dom = roofline_from_profile(profmx, a.np1)
     if roof is not None:
@@ -514,6 +534,7 @@ def main():
                   "instrumented_ms_per_step": round(tmx_i * 1e3, 4)},
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         "projection": line_proj,
+        "fgmres": fgm,
         "autotuned": {"fp64": tune64, "mixed": tunemx},
         "kernel_timing": "HIP events around every launch of the instrumented replay of the timed solves",
         "kernels_mixed": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
