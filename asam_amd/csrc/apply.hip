@@ -8,6 +8,7 @@
 // columns per wavefront (coalesced along x at every level), loads issued a window of
 // levels ahead of the recurrence, forward-sweep results staged in LDS for the
 // backward sweep.
+#include "../../include/asam_sound.h"
 #include "apply_tile.h"
 
 namespace asam {
@@ -236,57 +237,62 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
   }
 }
 
-// The fused-option combinations the solvers use.
+// The fused-option combinations the solvers use; any other set (or an x = 0 form with FP32
+// coefficients) launches nothing and returns ASAM_E_INVALID.
 template <int NP, int TR, typename CT, typename VX, typename VO, typename VU>
-static void apply_fl(hipStream_t s, const Geom& g, const Coef<CT>& c, const ApplyReq& r) {
+static int apply_fl(hipStream_t s, const Geom& g, const Coef<CT>& c, const ApplyReq& r) {
   int fl = (r.b ? AF_B : 0) | (r.y2 ? AF_Y2 : 0) | (r.dx ? AF_DX : 0) |
            ((r.dots & DOT_YY) ? AF_YY : 0) | ((r.dots & DOT_YU) ? AF_YU : 0) |
            (r.x0_dead ? AF_X0 : 0);
   switch (fl) {
     case AF_X0 | AF_B | AF_Y2 | AF_YY:
-      if constexpr (sizeof(CT) == 8)
+      if constexpr (sizeof(CT) == 8) {
         apply_t<NP, TR, AF_X0 | AF_B | AF_Y2 | AF_YY, CT, VX, VO, VU>(s, g, c, r);
-      break;
+        return 0;
+      }
+      return ASAM_E_INVALID;
     case AF_X0 | AF_B | AF_YY:
-      if constexpr (sizeof(CT) == 8)
+      if constexpr (sizeof(CT) == 8) {
         apply_t<NP, TR, AF_X0 | AF_B | AF_YY, CT, VX, VO, VU>(s, g, c, r);
-      break;
+        return 0;
+      }
+      return ASAM_E_INVALID;
     case AF_X0 | AF_B:
-      if constexpr (sizeof(CT) == 8) apply_t<NP, TR, AF_X0 | AF_B, CT, VX, VO, VU>(s, g, c, r);
-      break;
-    case 0: apply_t<NP, TR, 0, CT, VX, VO, VU>(s, g, c, r); break;
-    case AF_YU: apply_t<NP, TR, AF_YU, CT, VX, VO, VU>(s, g, c, r); break;
-    case AF_YY | AF_YU: apply_t<NP, TR, AF_YY | AF_YU, CT, VX, VO, VU>(s, g, c, r); break;
-    case AF_B: apply_t<NP, TR, AF_B, CT, VX, VO, VU>(s, g, c, r); break;
-    case AF_B | AF_YY: apply_t<NP, TR, AF_B | AF_YY, CT, VX, VO, VU>(s, g, c, r); break;
+      if constexpr (sizeof(CT) == 8) {
+        apply_t<NP, TR, AF_X0 | AF_B, CT, VX, VO, VU>(s, g, c, r);
+        return 0;
+      }
+      return ASAM_E_INVALID;
+    case 0: apply_t<NP, TR, 0, CT, VX, VO, VU>(s, g, c, r); return 0;
+    case AF_YU: apply_t<NP, TR, AF_YU, CT, VX, VO, VU>(s, g, c, r); return 0;
+    case AF_YY | AF_YU: apply_t<NP, TR, AF_YY | AF_YU, CT, VX, VO, VU>(s, g, c, r); return 0;
+    case AF_B: apply_t<NP, TR, AF_B, CT, VX, VO, VU>(s, g, c, r); return 0;
+    case AF_B | AF_YY: apply_t<NP, TR, AF_B | AF_YY, CT, VX, VO, VU>(s, g, c, r); return 0;
     case AF_B | AF_Y2 | AF_YY:
-      apply_t<NP, TR, AF_B | AF_Y2 | AF_YY, CT, VX, VO, VU>(s, g, c, r); break;
+      apply_t<NP, TR, AF_B | AF_Y2 | AF_YY, CT, VX, VO, VU>(s, g, c, r);
+      return 0;
     case AF_B | AF_Y2 | AF_DX | AF_YY:
-      apply_t<NP, TR, AF_B | AF_Y2 | AF_DX | AF_YY, CT, VX, VO, VU>(s, g, c, r); break;
+      apply_t<NP, TR, AF_B | AF_Y2 | AF_DX | AF_YY, CT, VX, VO, VU>(s, g, c, r);
+      return 0;
     default:
-      // every option set the solvers request is listed above
-      break;
+      return ASAM_E_INVALID;
   }
 }
 
 template <int NP, int TR>
-static void apply_np(hipStream_t s, const Geom& g, const Coef<double>& c64,
-                     const Coef<float>& c32, const ApplyReq& r) {
+static int apply_np(hipStream_t s, const Geom& g, const Coef<double>& c64,
+                    const Coef<float>& c32, const ApplyReq& r) {
   if (r.prec == 0) {
-    if (r.vo == 0) apply_fl<NP, TR, double, double, double, double>(s, g, c64, r);
-    else apply_fl<NP, TR, double, double, float, double>(s, g, c64, r);
-  } else {
-    apply_fl<NP, TR, float, float, float, float>(s, g, c32, r);
+    if (r.vo == 0) return apply_fl<NP, TR, double, double, double, double>(s, g, c64, r);
+    return apply_fl<NP, TR, double, double, float, double>(s, g, c64, r);
   }
+  return apply_fl<NP, TR, float, float, float, float>(s, g, c32, r);
 }
 
-void launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
-                  const ApplyReq& r) {
-  if (g.np1 == 1) {
-    if (r.tr) apply_np<1, 1>(s, g, c64, c32, r); else apply_np<1, 0>(s, g, c64, c32, r);
-  } else {
-    if (r.tr) apply_np<2, 1>(s, g, c64, c32, r); else apply_np<2, 0>(s, g, c64, c32, r);
-  }
+int launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
+                 const ApplyReq& r) {
+  if (g.np1 == 1) return r.tr ? apply_np<1, 1>(s, g, c64, c32, r) : apply_np<1, 0>(s, g, c64, c32, r);
+  return r.tr ? apply_np<2, 1>(s, g, c64, c32, r) : apply_np<2, 0>(s, g, c64, c32, r);
 }
 
 }  // namespace asam

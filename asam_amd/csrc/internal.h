@@ -126,8 +126,9 @@ int apply_blocks(const Geom& g);
 // dead[c] = 1 where the residual form writes 0 (no volume and no face coefficient), from
 // the FP64 coefficients with the apply's own test.
 void launch_dead_mask(hipStream_t s, const Geom& g, const Coef<double>& c64, unsigned char* dead);
-void launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
-                  const ApplyReq& r);
+// Returns 0, or ASAM_E_INVALID (nothing launched) for an option set no kernel is built for.
+int launch_apply(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
+                 const ApplyReq& r);
 
 // ------------------------------------------------------------------ line solve
 // out = -M^-1 in (PSolve1 semantics) with in formed on the fly:

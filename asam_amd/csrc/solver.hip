@@ -45,6 +45,7 @@ struct asam_sound_ctx {
   // (F)GMRES basis
   int gm_m = 0;
   std::vector<double*> gv, gz;
+  int launch_err = 0;  // an apply found no kernel for its option set (nothing launched)
   int gm32_m = 0;  // FP32 (F)GMRES basis of the mixed solve
   std::vector<float*> gv32, gz32;
   double* gtmp = nullptr;
@@ -410,7 +411,8 @@ struct Enq {
     r.tr = tr; r.prec = prec; r.vx = vx; r.vo = vo; r.vu = vu; r.x = x; r.y = y; r.b = b;
     r.y2 = y2; r.u = u; r.dots = dots; r.partial = c->partial; r.done = done; r.dx = dx;
     r.xw = xw;
-    timed(prec ? 1 : 0, apply_bytes(c->g, r), [&] { launch_apply(s, c->g, c->c64, c->c32, r); },
+    timed(prec ? 1 : 0, apply_bytes(c->g, r),
+          [&] { c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0; },
           tag);
     c->counts[0]++;
     if (fkind >= 0) fin(fkind, fnq, apply_blocks(c->g), 0, 0, tag);
@@ -431,9 +433,9 @@ struct Enq {
     ApplyReq r;
     r.tr = tr; r.prec = 0; r.vx = 0; r.vo = vo; r.vu = 0; r.x = x; r.y = y; r.b = b;
     r.y2 = y2; r.dots = dots; r.partial = c->partial; r.done = &c->st->xz;
-    launch_apply(s, c->g, c->c64, c->c32, r);
+    c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     r.x = nullptr; r.x0_dead = c->dead; r.done = &c->st->xnz;
-    launch_apply(s, c->g, c->c64, c->c32, r);
+    c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     c->counts[0]++;
     if (fkind >= 0) fin(fkind, 1, apply_blocks(c->g));
   }
@@ -914,6 +916,7 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
   if (!(*tol >= 0.0)) return fail(c, ASAM_E_INVALID, "tol");
   HIPCHK(c, hipSetDevice(c->device));
   memset(c->counts, 0, sizeof(c->counts));
+  c->launch_err = 0;
   asam_sound_info tmp;
   memset(&tmp, 0, sizeof tmp);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -930,6 +933,10 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
                           &tmp);
   }
   if (rc) return rc;
+  if (c->launch_err) {
+    c->launch_err = 0;
+    return fail(c, ASAM_E_INVALID, "an operator apply has no kernel for its option set");
+  }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   if (c->prof_on) prof_resolve(c, false);
@@ -986,8 +993,13 @@ extern "C" int asam_sound_apply_device(asam_sound_ctx* c, double* y, const doubl
   if (!c->ready) return fail(c, ASAM_E_STATE, "apply before setup");
   if (!x || !y || x == y) return fail(c, ASAM_E_INVALID, "x, y must be distinct buffers");
   HIPCHK(c, hipSetDevice(c->device));
+  c->launch_err = 0;
   Enq e(c, tr ? 1 : 0);
   e.apply(0, 0, 0, 0, x, y, nullptr, nullptr, nullptr, 0, nullptr);
+  if (c->launch_err) {
+    c->launch_err = 0;
+    return fail(c, ASAM_E_INVALID, "an operator apply has no kernel for its option set");
+  }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
