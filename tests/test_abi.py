@@ -67,3 +67,16 @@ def test_no_cpu_fallback_in_product(sound_mod):
             m.load(os.path.join(ROOT, "does_not_exist.so"))
         finally:
             m._lib = saved
+
+
+def test_method_constants_match_header(sound_mod):
+    """The Python and Fortran method codes equal the C header's ASAM_METHOD_* values."""
+    import re
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "asam_sound.h")).read()
+    c_codes = {m: int(v) for m, v in re.findall(r"#define ASAM_METHOD_(\w+) (\d+)", hdr)}
+    assert c_codes == sound_mod.METHODS
+    f90 = open(os.path.join(root, "asam_amd", "fortran", "asam_sound_iface.f90")).read()
+    f_codes = {m: int(v) for m, v in
+               re.findall(r"ASAM_METHOD_(\w+) = (\d+)", f90)}
+    assert f_codes == c_codes
