@@ -285,6 +285,30 @@ def test_config2_full_size_parity(np1, oracle_mod, sound_mod):
     assert rr <= 1.0001e-8
 
 
+@pytest.mark.parametrize("method", ["FGMRES", "FGMRESV"])
+def test_config2_full_size_fgmres(method, oracle_mod, sound_mod):
+    """The reference's default method and its classical Gram-Schmidt variant on the
+    256x256x64 MultiTriTB system at its real size: FP64 vs the oracle (steps +-1, 1e-10),
+    and the mixed solve to the 1e-8 residual of the FP64 system."""
+    p = W.config(2, np1=2)
+    orc = oracle_mod.Oracle(p)
+    f = orc.fgmresv if method == "FGMRESV" else orc.fgmres
+    xo, ito, rro = f(p.b, 20, 1e-8, 10)
+    s = sound_mod.SoundSolver.from_problem(p)
+    s.set_control(method=method, step_gmres=10)
+    x, mi, tl, info = s.SolveSound(np.zeros(p.n), p.b, 20, 1e-8)
+    e = rel(x, xo)
+    print(f"cfg2 {method} FP64: oracle steps {ito}, gpu {info.iterations}, diff {e:.3e}, "
+          f"{info.ms:.2f} ms")
+    assert abs(info.iterations - ito) <= 1 and e < SOL_TOL
+    s.set_control(method=method, step_gmres=10, precision="MIXED")
+    xm, _, _, im = s.SolveSound(np.zeros(p.n), p.b, 20, 1e-8)
+    rr = np.linalg.norm(p.b - orc.matvec1(xm)) / np.linalg.norm(p.b)
+    print(f"cfg2 {method} mixed: steps {im.iterations} refinements {im.restarts} true rel res "
+          f"{rr:.3e} err vs FP64 oracle {rel(xm, xo):.3e}, {im.ms:.2f} ms")
+    assert im.status == 0 and rr <= 1.0001e-8
+
+
 def test_device_dot_matches_fsum(sound_mod):
     """Two-stage reduction (DPP wavefront sums, LDS across waves, fixed-order finalize)
     against a correctly rounded sum, within the FP64 bound n u sum|x_i y_i|, on lengths
