@@ -824,7 +824,8 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (info) {
     info->iterations = hs->inner_total;
-    info->restarts = steps;
+    // without convergence the last residual's tail also set up a step that did not run
+    info->restarts = converged ? steps : (steps > 0 ? steps - 1 : 0);
     info->status = converged ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
     info->abs_residual = hs->outer_r;
     info->rel_residual = hs->outer_r0 > 0 ? hs->outer_r / hs->outer_r0 : 0.0;
@@ -900,7 +901,8 @@ static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, c
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (info) {
     info->iterations = hs->inner_total;
-    info->restarts = hs->outer;
+    // without convergence the last residual's tail also set up a step that did not run
+    info->restarts = converged ? hs->outer : (hs->outer > 0 ? hs->outer - 1 : 0);
     info->status = converged ? ASAM_CONVERGED : ASAM_NOT_CONVERGED;
     info->abs_residual = hs->outer_r;
     info->rel_residual = hs->outer_r0 > 0 ? hs->outer_r / hs->outer_r0 : 0.0;

@@ -183,6 +183,24 @@ def test_mixed_precision_reaches_1e8(np1, method, oracle_mod, sound_mod):
         assert err < 1e-6
 
 
+@pytest.mark.parametrize("method", ["BICGSTAB", "FGMRES", "FGMRESV"])
+def test_mixed_refinement_cap(method, oracle_mod, sound_mod):
+    """With one refinement step allowed the mixed solve stops there and says it did not
+    converge; the step it took still reduced the FP64 residual, and the returned x is that
+    step's iterate."""
+    p = W.config(4, np1=2, scale=0.5)
+    orc = oracle_mod.Oracle(p)
+    s = sound_mod.SoundSolver.from_problem(p)
+    s.set_control(method=method, precision="MIXED", step_gmres=10, inner_tol=1e-3,
+                  max_outer=1)
+    x, mi, tl, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-12)
+    rr = np.linalg.norm(p.b - orc.matvec1(x)) / np.linalg.norm(p.b)
+    print(f"{method} capped at one refinement step: status {info.status}, steps "
+          f"{info.restarts}, device rel residual {info.rel_residual:.3e}, true {rr:.3e}")
+    assert info.status == 1 and info.restarts == 1
+    assert rr < 1e-2 and abs(rr - info.rel_residual) <= 1e-6 * rr + 1e-15
+
+
 @pytest.mark.parametrize("method,precision", [("BICGSTAB", "FP64"), ("BICGSTAB", "MIXED"),
                                               ("FGMRES", "FP64"), ("FGMRESV", "FP64"),
                                               ("GMRES", "FP64")])
