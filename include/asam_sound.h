@@ -79,7 +79,8 @@ typedef struct {
   int iterations;         /* BiCGStab: completed iterations (a half-step exit counts as one);
                              (F)GMRES: Arnoldi steps summed over restarts;
                              mixed: inner iterations summed over refinement steps */
-  int restarts;           /* (F)GMRES restart cycles / mixed refinement steps */
+  int restarts;           /* (F)GMRES restart cycles / mixed: refinement steps run (FP32
+                             inner solves or cycles) */
   int status;             /* ASAM_CONVERGED / ASAM_NOT_CONVERGED / ASAM_BREAKDOWN */
   int reserved;
   double rel_residual;    /* final ||r|| / ||r0|| as tracked by the method */
