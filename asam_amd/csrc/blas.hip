@@ -117,15 +117,33 @@ __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
   if (!self) sk = st->vscale[k];
   double acc[1] = {0.0};
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    double wv = (double)w[e];
-    if (k > 0) {
-      wv = wv + (-h) * ((double)vprev[e] * sp);
-      w[e] = (VT)wv;
-      wv = (double)(VT)wv;
+  // U turns of the grid-stride loop at a time: the loads of all U elements first, then
+  // their arithmetic in the plain loop's order (same elements, same sums, same bits)
+  constexpr int U = 4;
+  for (long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x; e0 < n; e0 += U * stride) {
+    VT wl[U], pl[U], kl[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long e = e0 + u * stride;
+      const bool ok = e < n;
+      wl[u] = ok ? w[e] : (VT)0;
+      pl[u] = (ok && k > 0) ? vprev[e] : (VT)0;
+      kl[u] = (ok && !self) ? vk[e] : (VT)0;
     }
-    if (self) acc[0] += wv * wv;
-    else acc[0] += ((double)vk[e] * sk) * wv;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long e = e0 + u * stride;
+      if (e < n) {
+        double wv = (double)wl[u];
+        if (k > 0) {
+          wv = wv + (-h) * ((double)pl[u] * sp);
+          w[e] = (VT)wv;
+          wv = (double)(VT)wv;
+        }
+        if (self) acc[0] += wv * wv;
+        else acc[0] += ((double)kl[u] * sk) * wv;
+      }
+    }
   }
   block_reduce_store_b<1>(acc, partial, blockIdx.x);
 }
@@ -195,11 +213,26 @@ __global__ __launch_bounds__(256) void k_cgs_dots(long n, const VT* __restrict__
     v[q] = q < nq ? (const VT*)vl.z[k0 + q] : w;
   }
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    const double wv = (double)w[e];
+  constexpr int U = 4;  // U grid-stride turns at a time, loads first (same order and sums)
+  for (long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x; e0 < n; e0 += U * stride) {
+    VT wl[U], vl4[4][U];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (q < nq) acc[q] += ((double)v[q][e] * sk[q]) * wv;
+    for (int u = 0; u < U; ++u) {
+      const long e = e0 + u * stride;
+      const bool ok = e < n;
+      wl[u] = ok ? w[e] : (VT)0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) vl4[q][u] = (ok && q < nq) ? v[q][e] : (VT)0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (e0 + u * stride < n) {
+        const double wv = (double)wl[u];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (q < nq) acc[q] += ((double)vl4[q][u] * sk[q]) * wv;
+      }
+    }
   }
   block_reduce_store_b<4>(acc, partial, blockIdx.x);
 }
