@@ -32,6 +32,8 @@
 namespace asam {
 
 size_t lds_budget();
+size_t lds_budget_for(int line_lds);
+int lds_levels(int nz, int np, size_t word, size_t budget);
 
 struct EnsMember {
   int nx, ny, nz, np1;
@@ -130,6 +132,7 @@ struct KEnsLine {
   double* partial;
   long pstride;
   int qslot;
+  int kl;  // levels staged in LDS (the rest through the output vector)
 };
 
 template <int NP, int MODE, int LDS, int P, typename CT, typename VT>
@@ -154,6 +157,7 @@ __global__ __launch_bounds__(64) void k_ens_line(KEnsLine<CT, VT> e) {
     a.v = e.v ? e.v + v : nullptr;
     a.out = e.out + v;
     a.scale = nullptr; a.st = st; a.partial = nullptr; a.done = nullptr; a.qslot = 0;
+    a.kl = e.kl;
     const int lt = g - M.lt0;
     double acc[1] = {0.0};
     line_tile<NP, 0, MODE, LDS, P, CT, VT>(a, lt % M.lgx, lt / M.lgx, M.lgx, stage, acc);
@@ -436,6 +440,7 @@ static int valid_tuning(const asam_ens_tuning& t) {
   if (t.apply_pf < 0 || t.apply_pf > 64 || t.line_pf < 0 || t.line_pf > 64) return 0;
   if (t.blas_pf < 0 || t.blas_pf > 64) return 0;
   if (t.line_window != 4 && t.line_window != 8) return 0;
+  if (t.line_lds < 0 || t.line_lds > 160) return 0;
   if (t.blas_chunk < 1024 || t.blas_chunk > 65536 || (t.blas_chunk & 255)) return 0;
   return 1;
 }
@@ -566,8 +571,9 @@ void line_inner(asam_ens_ctx* c, const float* src, float* p, const float* v, flo
   e.c = c->c32; e.mem = c->md; e.tmem = c->tL; e.ntiles = c->nL;
   e.src = src; e.p = p; e.v = v; e.out = out; e.st = c->st;
   e.partial = c->partial; e.pstride = c->pstride; e.qslot = qslot;
-  const size_t lds = (size_t)c->maxnz * 64 * NP * sizeof(float);
-  const bool use = c->tune.line_lds && lds <= lds_budget();
+  e.kl = lds_levels(c->maxnz, NP, sizeof(float), lds_budget_for(c->tune.line_lds));
+  const size_t lds = (size_t)e.kl * 64 * NP * sizeof(float);
+  const bool use = e.kl > 0;
   if (use) {
     if (c->tune.line_window == 8) line_go<NP, MODE, 1, 8>(c, e, lds);
     else line_go<NP, MODE, 1, 4>(c, e, lds);

@@ -35,7 +35,8 @@ struct Tuning {
   int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread
   int apply_wg = 4096;    // target workgroups per apply launch
   int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
-  int line_lds = 1;       // stage forward-sweep results in LDS (else in the output vector)
+  int line_lds = 1;       // LDS staging budget of the line solve's forward sweep per
+                          // workgroup: 0 none (output vector), 1 default, n >= 2 n KiB
   int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
   int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)

@@ -8,19 +8,41 @@ namespace asam {
 // ===================================================================== line solve
 int line_blocks(const Geom& g) { return ((g.nx + 63) / 64) * g.ny; }
 
-// Dynamic LDS a line-solve workgroup may take: the device's per-workgroup limit
-// (capped at 64 KiB) minus 4 KiB kept for the static reduction scratch.
-size_t lds_budget() {
+// Dynamic LDS a line-solve workgroup may take: the device's per-workgroup limit minus
+// 4 KiB kept for the static reduction scratch.
+size_t lds_device_limit() {
   static int cached = -1;
   if (cached < 0) {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       v = 0;
-    if (v > 64 * 1024) v = 64 * 1024;
     cached = v > 4096 ? v - 4096 : 0;
   }
   return (size_t)cached;
+}
+
+// Default staging budget per workgroup: at most 60 KiB, so that two or more one-wavefront
+// workgroups share a CU.
+size_t lds_budget() {
+  const size_t d = lds_device_limit();
+  return d < 60 * 1024 ? d : 60 * 1024;
+}
+
+// Staging budget the line_lds tuning knob names: 0 none, 1 the default, n >= 2 n KiB
+// (never more than the device allows).
+size_t lds_budget_for(int line_lds) {
+  if (line_lds <= 0) return 0;
+  if (line_lds == 1) return lds_budget();
+  const size_t want = (size_t)line_lds * 1024, d = lds_device_limit();
+  return want < d ? want : d;
+}
+
+// Levels of a column of nz levels staged in LDS within `budget` bytes.
+int lds_levels(int nz, int np, size_t word, size_t budget) {
+  const size_t per = (size_t)64 * np * word;
+  const size_t k = budget / per;
+  return k >= (size_t)nz ? nz : (int)k;
 }
 
 // One 64-column tile per workgroup: grid (gxl, ny).
@@ -61,8 +83,9 @@ static void line_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const LineRe
   a.src = (const VT*)r.src; a.p = (VT*)r.p; a.v = (const VT*)r.v; a.out = (VT*)r.out;
   a.scale = r.scale; a.st = r.st; a.partial = r.partial; a.done = r.done;
   a.qslot = r.qslot;
-  const size_t lds = (size_t)g.nz * 64 * NP * sizeof(CT);
-  const bool use_lds = r.use_lds && g.tune.line_lds && lds <= lds_budget();
+  a.kl = r.use_lds ? lds_levels(g.nz, NP, sizeof(CT), lds_budget_for(g.tune.line_lds)) : 0;
+  const size_t lds = (size_t)a.kl * 64 * NP * sizeof(CT);
+  const bool use_lds = a.kl > 0;
   // default window: 8 levels, 4 for FP64 2x2 blocks (register budget)
   const int P = g.tune.line_window > 0 ? g.tune.line_window
                                        : ((sizeof(CT) == 8 && NP == 2) ? 4 : 8);

@@ -20,6 +20,7 @@ struct KLine {
   double* partial;
   const int* done;
   int qslot;
+  int kl;  // LDS staging: levels k < kl are staged in LDS, the rest in the output vector
 };
 
 template <typename T> struct M4 { T m[4]; };
@@ -105,21 +106,28 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
       if constexpr (NP == 1) return V{m.m[0] * v.a, (CT)0};
       else return mtv(m, v);
     };
+    // Forward-sweep results: the lowest kl levels in LDS (written first, read last, so
+    // the ones a round trip through memory would most likely miss in the caches), the
+    // levels above in the output vector (their backward reads follow the writes closely).
+    // k is the same in every lane, so the test is a uniform branch.
     auto stage_put = [&](int k, long c, V y) {
       if constexpr (LDS) {
-        stage[(NP * k) * 64 + tid] = y.a;
-        if constexpr (NP == 2) stage[(2 * k + 1) * 64 + tid] = y.b;
-      } else {
-        stv(OUT, c, y);
+        if (k < a.kl) {
+          stage[(NP * k) * 64 + tid] = y.a;
+          if constexpr (NP == 2) stage[(2 * k + 1) * 64 + tid] = y.b;
+          return;
+        }
       }
+      stv(OUT, c, y);
     };
     auto stage_get = [&](int k, long c) -> V {
       if constexpr (LDS) {
-        if constexpr (NP == 1) return V{stage[k * 64 + tid], (CT)0};
-        else return V{stage[(2 * k) * 64 + tid], stage[(2 * k + 1) * 64 + tid]};
-      } else {
-        return ldv((const VT*)OUT, c);
+        if (k < a.kl) {
+          if constexpr (NP == 1) return V{stage[k * 64 + tid], (CT)0};
+          else return V{stage[(2 * k) * 64 + tid], stage[(2 * k + 1) * 64 + tid]};
+        }
       }
+      return ldv((const VT*)OUT, c);
     };
     // --- forward sweep: y_k = t_k^-1 (in_k - l_k y_k-1)   (transposed: in_k - w_k-1^T y_k-1)
     V yprev{0, 0};

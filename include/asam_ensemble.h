@@ -35,7 +35,8 @@ typedef struct {
   int apply_zsplit;  /* tiles per column along z (1..8): each member's nz split evenly */
   int apply_pf;      /* workgroups per CU of the persistent apply grid; 0: one per tile */
   int line_window;   /* levels per load window of the line solve: 4 or 8 */
-  int line_lds;      /* stage the forward sweep in LDS (1) or in the output vector (0) */
+  int line_lds;      /* LDS budget of the line solve's forward sweep per workgroup: 0 none
+                        (output vector), 1 default (60 KiB), 2..160 that many KiB */
   int line_pf;       /* workgroups per CU of the persistent line grid; 0: one per tile */
   int blas_chunk;    /* vector elements per tile of the update kernels: 1024..65536 */
   int blas_pf;       /* workgroups per CU of the persistent update grid; 0: one per tile */
