@@ -28,6 +28,8 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       st->status = 1;
       if (st->rho == 0.0) {  // reference: MaxIter = Iter, Tol = 0
         st->result = 1; st->rnorm = 0.0; st->status = 2; st->done = 1; st->pad = 1;
+      } else if (!isfinite(st->rho)) {  // non-finite b or x0: stop, ||r|| stays non-finite
+        st->result = 2; st->status = 1; st->done = 1; st->pad = 1;
       }
       break;
     }
@@ -68,7 +70,8 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       st->rnorm = rn;
       if (rn <= st->tol * st->rnorm0) {
         st->result = st->iter; st->status = 0; st->done = 1;
-      } else if (st->iter == st->maxiter) {
+      } else if (st->iter == st->maxiter || !isfinite(rn)) {
+        // cap reached, or a non-finite iterate (rnorm keeps it for the caller)
         st->result = 2 * st->iter; st->status = 1; st->done = 1;
       } else {
         st->rho_old = st->rho;
@@ -91,6 +94,10 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       st->gbeta = beta;
       if (beta <= 0.0) {
         st->done = 1; st->pad = 1; st->cycle_done = 1; st->grho = 0.0; st->gstatus = 0;
+        break;
+      }
+      if (!isfinite(beta)) {  // non-finite residual: no cycle, grho reports it
+        st->done = 1; st->pad = 1; st->cycle_done = 1; st->grho = beta; st->gstatus = 1;
         break;
       }
       st->vscale[0] = 1.0 / beta;
@@ -132,6 +139,10 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       if (st->grho <= st->gtol * st->grho0) {
         st->nR = i + 1;
         st->cycle_done = 1;
+      } else if (!isfinite(st->grho)) {  // non-finite step: end the cycle and the solve
+        st->nR = i + 1;
+        st->cycle_done = 1;
+        st->done = 1;
       } else {
         st->nR = st->m;
         if (i == st->m - 1) st->cycle_done = 1;
@@ -153,8 +164,9 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       double r = sqrt(S[0]);
       if (st->outer == 0) st->outer_r0 = r;
       st->outer_r = r;
-      if (r <= st->outer_tol * st->outer_r0 || r == 0.0) {
-        st->outer_done = 1; st->gstatus = 0; st->done = 1; st->pad = 1;
+      if (r <= st->outer_tol * st->outer_r0 || r == 0.0 || !isfinite(r)) {
+        // converged, or a non-finite residual (outer_r reports it; no further step)
+        st->outer_done = 1; st->gstatus = isfinite(r) ? 0 : 1; st->done = 1; st->pad = 1;
         break;
       }
       // Ask the inner solve for inner_tol, but no more than the outer target needs.
@@ -183,8 +195,9 @@ __device__ inline void fin_logic(int kind, const double* S, KState* st, int k, i
       const double r = sqrt(S[0]);
       if (st->outer == 0) st->outer_r0 = r;
       st->outer_r = r;
-      if (r <= st->outer_tol * st->outer_r0 || r == 0.0) {
-        st->outer_done = 1; st->gstatus = 0; st->done = 1; st->pad = 1; st->cycle_done = 1;
+      if (r <= st->outer_tol * st->outer_r0 || r == 0.0 || !isfinite(r)) {
+        st->outer_done = 1; st->gstatus = isfinite(r) ? 0 : 1; st->done = 1; st->pad = 1;
+        st->cycle_done = 1;
         break;
       }
       double need = 0.5 * st->outer_tol * st->outer_r0 / r;

@@ -54,6 +54,7 @@ struct asam_sound_ctx {
   int last_iters = 4;     // BiCGStab iterations of the last plain solve (first chunk size)
   int inner_hist[32] = {0};  // inner iterations per refinement step of the last mixed solve
   int mixed_steps = 0;       // refinement steps of the last converged mixed solve (0: unknown)
+  int mixed_cap = 0;         // inner iteration cap that solve ran with
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
   // live per-kernel timing (asam_sound_profile_enable)
@@ -142,7 +143,8 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.apply_variant = t->apply_variant;
   nt.apply_wg = t->apply_wg;
   nt.line_window = t->line_window;
-  nt.line_lds = t->line_lds ? 1 : 0;
+  if (t->line_lds < 0 || t->line_lds > 160) return fail(c, ASAM_E_INVALID, "line_lds");
+  nt.line_lds = t->line_lds;
   if (t->fused_tail) return fail(c, ASAM_E_INVALID, "fused_tail is not supported");
   if (t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
     return fail(c, ASAM_E_INVALID, "apply_bx");
@@ -754,15 +756,18 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classic
 // that many steps is enqueued at once and synchronised once: kernels after convergence
 // return at once on the device flags, and an inner solve the schedule cuts short only
 // makes the next refinement step start from a less refined x.  Whatever is not finished
-// then continues step by step.
+// then continues step by step.  The caller's maxiter (the BiCGStab iteration cap of the
+// FP64 solve) caps every inner solve, together with the control's max_inner.
 static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, double tol,
-                       asam_sound_info* info) {
+                       int maxiter, asam_sound_info* info) {
   Enq e(c, tr);
   KState* hs = c->st_host;
   memset(hs, 0, sizeof(KState));
+  const int max_inner = maxiter < c->ctl.max_inner ? maxiter : c->ctl.max_inner;
+  const int max_outer = c->ctl.max_outer;
   hs->outer_tol = tol;
   hs->inner_tol = c->ctl.inner_tol;
-  hs->maxiter = c->ctl.max_inner;
+  hs->maxiter = max_inner;
   hs->xz = 1;
   int rc = upload_state(c);
   if (rc) return rc;
@@ -780,12 +785,12 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   };
   int k = 0;  // last refinement step enqueued
   const int ns = c->mixed_steps;
-  if (!c->prof_on && ns > 0 && ns <= 16 && ns <= c->ctl.max_outer) {
+  if (!c->prof_on && ns > 0 && ns <= 16 && ns <= max_outer && c->mixed_cap == max_inner) {
     outer_step(0);
     for (k = 1; k <= ns; ++k) {
       inner_fill();
       const int h = c->inner_hist[k - 1] > 0 ? c->inner_hist[k - 1] : 1;
-      for (int q = 0; q < h && q < c->ctl.max_inner; ++q) {
+      for (int q = 0; q < h && q < max_inner; ++q) {
         e.iter = q + 1;
         bicg_iteration<float>(e, 1, c->d32, c->r32, c->rt32, c->p32, c->ph32, c->v32, c->rh32,
                               c->t32);
@@ -802,11 +807,11 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   bool converged = hs->outer_done != 0;
   // step-by-step continuation: k is the device's current step (its residual is known)
   k = hs->outer_done ? hs->outer : hs->outer - 1;
-  while (!converged && k < c->ctl.max_outer) {
+  while (!converged && k < max_outer) {
     inner_fill();
     int issued = 0;
     const int hist = k < 32 ? c->inner_hist[k] : 0;
-    rc = bicg_loop<float>(e, 1, c->ctl.max_inner, c->d32, c->r32, c->rt32, c->p32, c->ph32,
+    rc = bicg_loop<float>(e, 1, max_inner, c->d32, c->r32, c->rt32, c->p32, c->ph32,
                           c->v32, c->rh32, c->t32, &issued, hist > 0 ? hist : 4);
     if (rc) return rc;
     ++k;
@@ -818,7 +823,9 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
   // per-step inner counts as the device recorded them, for the next solve's schedule
   const int steps = hs->outer;
   for (int q = 0; q < steps && q < 16; ++q) c->inner_hist[q] = hs->hist[q] > 0 ? hs->hist[q] : 1;
-  c->mixed_steps = converged ? steps : 0;
+  // a solve that ended on a non-finite residual leaves no schedule
+  c->mixed_steps = (converged && isfinite(hs->outer_r)) ? steps : 0;
+  c->mixed_cap = max_inner;
   double* xfin = bufs[hs->xpar & 1];
   if (xfin != x) HIPCHK(c, hipMemcpyAsync(x, xfin, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -838,9 +845,12 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
 // vector) and one FP32 cycle of at most m Arnoldi steps on A d = r from d = 0, to the inner
 // tolerance; then x += d is fused into the next step's residual.  The reference's restart
 // (a new cycle from the current iterate, Iter_Mod.f90:136-139) is the next refinement step,
-// so every cycle starts from an FP64 residual.
+// so every cycle starts from an FP64 residual.  The caller's maxiter (restart cycles of
+// the FP64 solve) therefore caps the refinement steps, together with the control's
+// max_outer.
 static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, const double* b,
-                          double tol, asam_sound_info* info) {
+                          double tol, int maxiter, asam_sound_info* info) {
+  const int max_outer = maxiter < c->ctl.max_outer ? maxiter : c->ctl.max_outer;
   const int m = c->ctl.step_gmres;
   int rc = ensure_gmres32(c, m);
   if (rc) return rc;
@@ -870,7 +880,7 @@ static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, c
   rc = sync_state(c);
   if (rc) return rc;
   int k = 0;
-  while (!hs->outer_done && k < c->ctl.max_outer) {
+  while (!hs->outer_done && k < max_outer) {
     e.timed(5, 4.0 * n, [&] { launch_fill(e.s, n, 1, c->d32, 0.0); });
     for (int i = 0; i < m; ++i) {
       e.line(1, LINE_PLAIN, c->gv32[i], nullptr, nullptr, c->gz32[i], &st->vscale[i],
@@ -924,9 +934,10 @@ static int solve_device_impl(asam_sound_ctx* c, int tr, double* x, const double*
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   int rc;
   if (c->ctl.precision == ASAM_PREC_MIXED && c->ctl.method != ASAM_METHOD_BICGSTAB) {
-    rc = solve_mixed_gm(c, tr, c->ctl.method == ASAM_METHOD_FGMRESV, x, b, *tol, &tmp);
+    rc = solve_mixed_gm(c, tr, c->ctl.method == ASAM_METHOD_FGMRESV, x, b, *tol, *maxiter,
+                        &tmp);
   } else if (c->ctl.precision == ASAM_PREC_MIXED) {
-    rc = solve_mixed(c, tr, x, b, *tol, &tmp);
+    rc = solve_mixed(c, tr, x, b, *tol, *maxiter, &tmp);
   } else if (c->ctl.method == ASAM_METHOD_BICGSTAB) {
     rc = solve_bicg_fp64(c, tr, x, b, maxiter, tol, &tmp);
   } else {
@@ -1348,7 +1359,7 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
           if (rc) { c->g.tune = t0; return rc; }
           if (m < best_ms) { best_ms = m; bt = c->g.tune; }
         }
-  const int windows[] = {1, 4, 8}, ldss[] = {1, 0};
+  const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48};
   const Tuning ba = bt;
   for (int w : windows)
     for (int l : ldss) {

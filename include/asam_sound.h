@@ -94,7 +94,10 @@ typedef struct {
   int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread */
   int apply_wg;      /* target workgroups per stencil-apply launch (256..16384) */
   int line_window;   /* levels per load window of the line solve: 1, 4 or 8 */
-  int line_lds;      /* 1: stage the forward sweep in LDS, 0: in the output vector */
+  int line_lds;      /* LDS budget per workgroup for the line solve's forward sweep: 0 none
+                        (staged in the output vector), 1 default (60 KiB), 2..160 that
+                        many KiB; the lowest levels that fit are staged in LDS, the rest
+                        in the output vector */
   int fused_tail;    /* reserved, must be 0 (reductions end in a separate finalize launch) */
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
   int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
@@ -141,6 +144,12 @@ int asam_sound_setup_weights(asam_sound_ctx *ctx, const double *FU, const double
  * return for BiCGStab the reference's convention (-Iter half-step exit, Iter full-step
  * exit, 2*Iter not converged, Iter with tol=0 on breakdown), unchanged otherwise.
  * tol: relative tolerance in; for BiCGStab the final ||r|| out (reference semantics).
+ * With ASAM_PREC_MIXED, maxiter and tol come back unchanged and maxiter keeps its meaning
+ * as a cap: BiCGStab caps every FP32 inner solve at min(maxiter, control max_inner)
+ * iterations; FGMRES/FGMRESV run one restart cycle per refinement step, so maxiter caps
+ * the refinement steps at min(maxiter, control max_outer).
+ * A non-finite b, x0 or iterate ends the solve at once and returns ASAM_E_NAN (x then
+ * holds the last iterate); the context stays usable.
  * info may be NULL. */
 int asam_sound_solve(asam_sound_ctx *ctx, double *x, const double *b, int *maxiter,
                      double *tol, asam_sound_info *info);

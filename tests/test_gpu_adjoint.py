@@ -122,14 +122,14 @@ def test_rosenbrock_adjoint_config4(sound_mod):
     Jd, _, _ = run_J(sound_mod, p, dt, w, ds, nsteps, weights=R.face_weights(p, theta))
     an = float(np.dot(sbar, ds))
     print(f"dot-product test: <w,Phi ds> {Jd:.15e} <Phi^T w, ds> {an:.15e} rel {rel(Jd, an):.2e}")
-    assert rel(Jd, an) < 1e-10
+    assert rel(Jd, an) < 1e-12  # measured 5e-14
     # finite differences w.r.t. b (the stage forcing), eps = 1e-6
     eps = 1e-6
     Jp, _, _ = run_J(sound_mod, p, dt, w, src + eps * ds, nsteps, weights=R.face_weights(p, theta))
     Jm, _, _ = run_J(sound_mod, p, dt, w, src - eps * ds, nsteps, weights=R.face_weights(p, theta))
     fd = (Jp - Jm) / (2 * eps)
     print(f"FD d/db: {fd:.12e} adjoint {an:.12e} rel {rel(fd, an):.2e}")
-    assert rel(fd, an) < 1e-6
+    assert rel(fd, an) < 1e-9  # J is linear in b: only round-off, measured 6e-14
     # finite differences w.r.t. the background theta field (eps = 1e-6 relative)
     dth = W.normal(p.grid.ncell, 46).reshape(theta.shape)
     e = 1e-6 * 300.0
@@ -138,4 +138,4 @@ def test_rosenbrock_adjoint_config4(sound_mod):
     fd = (Jp - Jm) / (2 * e)
     an = float(np.dot(thbar, dth.reshape(-1)))
     print(f"FD d/dtheta: {fd:.12e} adjoint {an:.12e} rel {rel(fd, an):.2e}")
-    assert rel(fd, an) < 1e-4
+    assert rel(fd, an) < 1e-7  # central-difference truncation, measured 1e-9

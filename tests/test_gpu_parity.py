@@ -229,25 +229,29 @@ def test_zero_guess_residual_form(np1, method, precision, oracle_mod, sound_mod)
         assert ia.iterations == ib.iterations and ia.status == ib.status == 0
 
 
+@pytest.mark.parametrize("transpose", [False, True])
 @pytest.mark.parametrize("np1", [1, 2])
-def test_line_windows_agree(np1, oracle_mod, sound_mod):
-    """The FP64 solve matches the oracle for every load window of the line solve, with and
-    without LDS staging of the sweep, and the windows agree with each other to round-off
-    (a wrong preconditioner still converges, but to a solution that differs at the
-    tolerance level, about 1e-9 here)."""
+def test_line_windows_agree(np1, transpose, oracle_mod, sound_mod):
+    """The FP64 solve (forward and transposed) matches the oracle for every load window of
+    the line solve and every staging of its forward sweep: all levels in LDS, none (the
+    output vector), and a 16 KiB budget that stages the lowest levels in LDS and the rest
+    in the output vector.  The variants agree with each other to round-off (a wrong
+    preconditioner still converges, but to a solution that differs at the tolerance
+    level, about 1e-9 here)."""
     p = W.config(1, np1=np1, scale=0.5)
-    xo, mo, _, _ = oracle_mod.Oracle(p).bicgstab(p.b, 300, 1e-8)
+    xo, mo, _, _ = oracle_mod.Oracle(p).bicgstab(p.b, 300, 1e-8, transpose=transpose)
     s = sound_mod.SoundSolver.from_problem(p)
+    solve = s.SolveSound_T if transpose else s.SolveSound
     ref = None
     for win in (1, 4, 8):
-        for lds in (0, 1):
+        for lds in (0, 1, 16):
             s.set_tuning(line_window=win, line_lds=lds)
-            x, mg, tg, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
+            x, mg, tg, info = solve(np.zeros(p.n), p.b, 300, 1e-8)
             if ref is None:
                 ref = x.copy()
             d, e = rel(x, ref), rel(x, xo)
-            print(f"np1={np1} window {win} lds {lds}: vs window 1 {d:.2e}, vs oracle {e:.2e}, "
-                  f"MaxIter {mg} (oracle {mo})")
+            print(f"np1={np1} transpose={transpose} window {win} lds {lds}: vs window 1 "
+                  f"{d:.2e}, vs oracle {e:.2e}, MaxIter {mg} (oracle {mo})")
             assert info.status == 0
             assert d < 1e-13
             assert e < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
