@@ -4,8 +4,8 @@ Reads the counter_collection.csv of one `rocprofv3 --pmc FETCH_SIZE` run and one
 `rocprofv3 --pmc WRITE_SIZE` run of the same command, groups dispatches by kernel class
 (the classes the solver's live profiler uses), scales each counter by the factor a
 streaming copy of known size reads for it (FETCH_SIZE and WRITE_SIZE are KiB per
-dispatch), and writes asam_amd/traffic.json, which bench.py reports as
-roofline.traffic.
+dispatch), and stores them in asam_amd/traffic.json under the workload's name (entries of
+other workloads are kept), which bench.py reports as roofline.traffic.
 
     python -m asam_amd.traffic FETCH_DIR WRITE_DIR CALIB_FETCH_DIR CALIB_WRITE_DIR \
         --workload "..." --command "..."
@@ -71,7 +71,7 @@ def main():
     fetch_scale = kib / (sum(cf["copy"]) / len(cf["copy"]))
     write_scale = kib / (sum(cw["copy"]) / len(cw["copy"]))
     f, w = read_counter(a.fetch), read_counter(a.write)
-    out = {"_about": {"workload": a.workload, "command": a.command,
+    out = {"_about": {"command": a.command,
                       "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
                       "calibration": f"copy of {a.copy_bytes:.0f} bytes read and written"}}
     for cls in sorted(set(f) & set(w)):
@@ -83,8 +83,16 @@ def main():
                     "read_bytes": int(round(fk * fetch_scale * 1024)),
                     "write_bytes": int(round(wk * write_scale * 1024)),
                     "launches": len(f[cls])}
+    # one entry per workload; the other workloads' entries are kept
+    try:
+        with open(OUT) as fh:
+            allw = json.load(fh)
+    except (OSError, ValueError):
+        allw = {}
+    allw = {"workloads": dict(allw.get("workloads", {}))}
+    allw["workloads"][a.workload] = out
     with open(OUT, "w") as fh:
-        json.dump(out, fh, indent=1)
+        json.dump(allw, fh, indent=1)
     print(json.dumps(out, indent=1))
 
 

@@ -53,6 +53,12 @@ def parse():
                          "64-member ensemble, nx,ny~U{64..256}, nz~U{40..64}) and exit")
     ap.add_argument("--size-scale", type=float, default=1.0,
                     help="ensemble: horizontal size factor of the members (1: full size)")
+    ap.add_argument("--cfg3-only", action="store_true",
+                    help="measure only the 512x512x96 solves (tuned shapes, per-kernel "
+                         "roofline) and exit (profiler runs)")
+    ap.add_argument("--tuning", default=None, metavar="JSON",
+                    help="--cfg3-only: launch shapes per precision from this file (as the "
+                         "'autotuned' field prints them) instead of running the autotuner")
     return ap.parse_args()
 
 
@@ -141,70 +147,149 @@ def read_traffic(kernel, workload):
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("_about", {}).get("workload") != workload:
-        return None
-    e = t.get(kernel)
+    e = t.get("workloads", {}).get(workload, {}).get(kernel)
     return e["bytes_per_launch"] if e else None
 
 
-def traffic_workload(np1, tol):
+def traffic_workload(np1, tol, cfg=2):
+    if cfg == 3:
+        return f"bench.py 512x512x96 tuned solves: cut-cell, np1={np1}, tol {tol:g}"
     return f"bench.py default solve: 256x256x64 cut-cell, np1={np1}, tol {tol:g}"
 
 
+def host_cpu():
+    """CPU model and core counts of the host the baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(p, tol, seconds):
+    """The oracle's FP64 BiCGStab on the same system: one warm-up solve (also the reference
+    answer), then the median of at least 5 solves on min(usable cores, 16) threads, and the
+    median of 3 single-threaded solves."""
     from oracle import pyoracle
-    threads = min(os.cpu_count() or 1, 16)
+    cpu = host_cpu()
+    threads = min(cpu["usable_cpus"] or 1, 16)
     pyoracle.set_threads(threads)
     orc = pyoracle.Oracle(p)
-    t0 = time.perf_counter()
     x, mi, tl, st = orc.bicgstab(p.b, 500, tol)
-    t1 = time.perf_counter()
-    times = [t1 - t0]
-    while sum(times) < seconds and len(times) < 8:
+    times = []
+    while len(times) < 5 or (sum(times) < seconds and len(times) < 8):
         t0 = time.perf_counter()
         orc.bicgstab(p.b, 500, tol)
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
+    pyoracle.set_threads(1)
+    one = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.bicgstab(p.b, 500, tol)
+        one.append(time.perf_counter() - t0)
+    pyoracle.set_threads(threads)
+    med1 = float(np.median(one))
     return {"value": round(1.0 / med, 4), "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"{len(times)} full FP64 BiCGStab solves of the same 256x256x64 system "
-                      f"(C restatement of the reference algorithm, OpenMP over columns), "
-                      f"median {med:.2f} s"}, x, mi, orc
+                      f"after one warm-up (C restatement of the reference algorithm, OpenMP "
+                      f"over columns), median {med:.3f} s",
+            "one_core": {"value": round(1.0 / med1, 4), "unit": "solves/s", "cores": 1,
+                         "sample": f"3 single-threaded solves, median {med1:.3f} s"},
+            "cpu": cpu}, x, mi, orc
 
 
-def hbm_roofline(np1, tol, steps, with_cpu):
-    """The 512x512x96 system (25.2 M cells, working set well beyond the 256 MB Infinity
-    Cache): apply/line kernel bandwidth (isolated launches, HIP events, FP32 and FP64), the
-    FP32-inner / FP64-refined and plain FP64 solve times (each precision autotuned), and the
-    mixed solution's error against the FP64 GPU solve and against the FP64 oracle."""
+def load_tuning(path):
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("fp64"), t.get("mixed")
+
+
+def class_table(prof, steps):
+    """Per kernel class of an instrumented replay: device ms and launches per solve,
+    algorithmic GB/s and its fraction of the HBM peak."""
+    out = {}
+    for k, v in prof.items():
+        if not v["launches"]:
+            continue
+        gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else None
+        out[k] = {"ms_per_solve": round(v["ms"] / steps, 4),
+                  "launches_per_solve": v["launches"] / steps,
+                  "avg_launch_us": round(1e3 * v["ms"] / v["launches"], 2),
+                  "GBs": round(gbs, 1) if gbs else None,
+                  "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    return out
+
+
+def cfg3_leg(np1, tol, steps, with_cpu, tuning=None):
+    """The 512x512x96 system (25.2 M cells; one FP32 vector is 100.7 MB, so the working set
+    is far beyond the 256 MB Infinity Cache and the kernels stream from HBM).  Each
+    precision gets the autotuner's launch shapes (or those of `tuning`); then K timed solves
+    and the same K solves replayed with HIP events around every launch on the solver's
+    stream give the solve times and, per kernel class, the in-solve algorithmic GB/s.  The
+    roofline entry is the dominant class of the mixed solve, with its HBM bytes per launch
+    from the rocprofv3 counter runs when those were made on this workload.  Isolated
+    launches of the default shapes are kept for comparison (not with `tuning`, the mode the
+    profiler runs use, whose traces then hold only the solves)."""
     from asam_amd import sound, workload
     p = workload.config(3, np1=np1)
     s = sound.SoundSolver.from_problem(p)
     N = p.grid.ncell
-    out = {}
-    # isolated kernels first, with the default launch shapes
-    for which, name in ((0, "apply"), (1, "line")):
+    iso = {}
+    for which, name in (((0, "apply"), (1, "line")) if tuning is None else ()):
         for fp32 in (True, False):
             ms = s.time_kernel(which, fp32, reps=20)
             w = 4 if fp32 else 8
             words = (7 if np1 == 1 else 13) if which == 0 else (5 if np1 == 1 else 16)
             gbs = words * w * N / (ms * 1e-3) / 1e9
-            out[f"{name}_{'fp32' if fp32 else 'fp64'}"] = {
+            iso[f"{name}_{'fp32' if fp32 else 'fp64'}"] = {
                 "avg_launch_us": round(ms * 1e3, 1), "achieved_GBs": round(gbs, 1),
                 "frac": round(gbs / HBM_PEAK_GBS, 4)}
     xbuf, bbuf = s.vec(), s.vec(p.b)
-    solves = {}
-    for prec in ("FP64", "MIXED"):
+    given = load_tuning(tuning) if tuning else (None, None)
+    solves, tuned, kern = {}, {}, {}
+    for prec, tg in (("FP64", given[0]), ("MIXED", given[1])):
         s.set_control(precision=prec)
-        s.autotune(3)
+        if tg:
+            s.set_tuning(**tg)
+            tuned[prec.lower()] = s.get_tuning()
+        else:
+            tuned[prec.lower()] = s.autotune(3)[0]
         t, infos, _ = timed_solves(s, xbuf, bbuf, steps, 1, tol, False)
-        solves[prec] = (t, infos[-1], xbuf.download())
+        ti, _, prof = timed_solves(s, xbuf, bbuf, steps, 1, tol, True)
+        solves[prec] = (t, infos[-1], xbuf.download(), prof, ti)
+        kern[prec.lower()] = class_table(prof, steps)
     x64, xmx = solves["FP64"][2], solves["MIXED"][2]
     rbuf = s.vec()
     s.apply_device(rbuf, xbuf)
     res = float(np.linalg.norm(p.b - rbuf.download()) / np.linalg.norm(p.b))
+    roof, dom = roofline_from_profile(solves["MIXED"][3], np1)
+    if roof is not None:
+        tr = read_traffic(dom, traffic_workload(np1, tol, 3))
+        roof["traffic"] = tr
+        roof["traffic_over_algorithmic"] = (round(tr / roof["bytes_per_launch"], 4)
+                                            if tr else None)
+    out = {"workload": f"512x512x96 cut-cell {'MultiTriTB' if np1 == 2 else 'MultiTriT'}, "
+                       f"ridge 2 km, BiCGStab + vertical-line precond, tol {tol:g}, x = 0",
+           "cells": N, "unknowns": p.n,
+           "roofline": roof,
+           "kernels_mixed": kern["mixed"], "kernels_fp64": kern["fp64"],
+           "autotuned": tuned,
+           "isolated_default_shapes": iso}
     out["solve"] = {
         "mixed_ms": round(solves["MIXED"][0] * 1e3, 3),
         "fp64_ms": round(solves["FP64"][0] * 1e3, 3),
+        "mixed_instrumented_ms": round(solves["MIXED"][4] * 1e3, 3),
+        "fp64_instrumented_ms": round(solves["FP64"][4] * 1e3, 3),
         "speedup_mixed_vs_fp64": round(solves["FP64"][0] / solves["MIXED"][0], 3),
         "mixed_inner_iterations": solves["MIXED"][1]["iterations"],
         "fp64_iterations": solves["FP64"][1]["iterations"],
@@ -212,11 +297,12 @@ def hbm_roofline(np1, tol, steps, with_cpu):
         "mixed_rel_diff_vs_fp64_gpu": float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))}
     if with_cpu:
         from oracle import pyoracle
-        pyoracle.set_threads(min(os.cpu_count() or 1, 16))
+        pyoracle.set_threads(min(host_cpu()["usable_cpus"] or 1, 16))
         orc = pyoracle.Oracle(p)
         t0 = time.perf_counter()
         xo, mo, to, so = orc.bicgstab(p.b, 500, tol)
         out["solve"]["oracle_fp64_s"] = round(time.perf_counter() - t0, 2)
+        out["solve"]["oracle_maxiter_out"] = mo
         out["solve"]["mixed_err_vs_fp64_oracle"] = float(np.linalg.norm(xmx - xo) /
                                                          np.linalg.norm(xo))
         out["solve"]["fp64_err_vs_fp64_oracle"] = float(np.linalg.norm(x64 - xo) /
@@ -224,6 +310,54 @@ def hbm_roofline(np1, tol, steps, with_cpu):
         del orc
     s.close()
     return out
+
+
+def varying_b_leg(s, p, steps, tol):
+    """The mixed solve as a time integration calls it: every solve gets a different b
+    (seeded N(0,1) on live cells), so the refinement schedule the context replays from its
+    previous solve is a prediction, not a copy.  Then the operator is set up again from a
+    perturbed state (theta + 0.05 K noise, the next step's system) and the varying-b solves
+    are timed on it; the first solve after the new set-up is timed on its own."""
+    from asam_amd import workload
+    act = np.repeat(p.VolC.reshape(-1) > 0.0, p.np1)
+    bs = [s.vec(np.where(act, workload.normal(p.n, 5000 + k), 0.0)) for k in range(steps)]
+    xbuf = s.vec()
+    s.set_control(precision="MIXED")
+
+    def run():
+        ts, its, worst = [], [], 0.0
+        for bb in bs:
+            s.zero(xbuf)
+            t0 = time.perf_counter()
+            _, _, info = s.solve_device(xbuf, bb, 500, tol)
+            ts.append(time.perf_counter() - t0)
+            its.append((info.restarts, info.iterations))
+            worst = max(worst, info.rel_residual)
+        return ts, its, worst
+
+    s.zero(xbuf)
+    s.solve_device(xbuf, bs[-1], 500, tol)  # the context's last solve: another b
+    ts, its, worst = run()
+    th = p.rhotheta / p.rho + 0.05 * workload.normal(p.grid.ncell, 77).reshape(p.rho.shape)
+    s.setup(p.FU, p.FV, p.FW, p.VolC, p.dxv, p.dyv, p.dzv, p.rho, p.rho * th, p.T)
+    s.zero(xbuf)
+    t0 = time.perf_counter()
+    _, _, info0 = s.solve_device(xbuf, bs[0], 500, tol)
+    first = time.perf_counter() - t0
+    ts2, its2, worst2 = run()
+    s.setup(p.FU, p.FV, p.FW, p.VolC, p.dxv, p.dyv, p.dzv, p.rho, p.rhotheta, p.T)
+    for b in bs:
+        b.free()
+    return {"workload": f"{steps} different seeded b on the same operator, then on an operator "
+                        f"set up from theta + 0.05 K noise; MIXED BiCGStab, tol {tol:g}, x = 0",
+            "ms_per_solve": round(1e3 * float(np.mean(ts)), 4),
+            "ms_min_max": [round(1e3 * min(ts), 4), round(1e3 * max(ts), 4)],
+            "refinements_inner_min_max": [list(min(its)), list(max(its))],
+            "worst_rel_residual": worst,
+            "perturbed_operator": {"ms_per_solve": round(1e3 * float(np.mean(ts2)), 4),
+                                   "first_solve_after_setup_ms": round(1e3 * first, 4),
+                                   "first_solve_refinements": info0.restarts,
+                                   "worst_rel_residual": worst2}}
 
 
 def ensemble_bench(a):
@@ -431,12 +565,12 @@ def transport_leg(steps, with_cpu, cpu_seconds=15.0, ncomp=4, tol=1e-6, maxiter=
     del s
     if with_cpu:
         from oracle import pyoracle, pytransport
-        threads = min(os.cpu_count() or 1, 16)
+        threads = min(host_cpu()["usable_cpus"] or 1, 16)
         pyoracle.set_threads(threads)
         o = pytransport.TransportOracle(tp)
         ct = []
         xo = None
-        while not ct or (sum(ct) < cpu_seconds and len(ct) < 3):
+        while len(ct) < 5:
             t0 = time.perf_counter()
             xo, mio, tlo, sto = o.bicgstab(tp.b, maxiter, tol)
             ct.append(time.perf_counter() - t0)
@@ -446,7 +580,8 @@ def transport_leg(steps, with_cpu, cpu_seconds=15.0, ncomp=4, tol=1e-6, maxiter=
                                "sample": f"{len(ct)} full FP64 transport solves of the same "
                                          f"system (C restatement; Gauss-Seidel single-threaded "
                                          f"as in the reference, products and BLAS on {threads} "
-                                         f"threads), median {cmed:.2f} s"}
+                                         f"threads), median {cmed:.2f} s",
+                               "cpu": host_cpu()}
         out["oracle_maxiter_out"] = mio
         inner = tp.interior(x).ravel()
         innero = tp.interior(xo).ravel()
@@ -472,6 +607,10 @@ def main():
     if a.transport_only:
         print(json.dumps({"transport": transport_leg(max(2, a.steps // 4), not a.no_cpu,
                                                      a.cpu_seconds)}))
+        return
+    if a.cfg3_only:
+        print(json.dumps({"roofline_cfg3": cfg3_leg(a.np1, a.tol, a.steps, not a.no_cpu,
+                                                    a.tuning)}))
         return
     from asam_amd import sound, workload
     p = workload.config(2, np1=a.np1)
@@ -503,6 +642,7 @@ def main():
 
     line_proj = projection_leg(s, p, max(2, a.steps // 4), a.tol)
     fgm = fgmres_leg(s, xbuf, bbuf, max(2, a.steps // 4), a.tol)
+    vary = varying_b_leg(s, p, max(4, a.steps // 2), a.tol)
 
     roof, dom = roofline_from_profile(profmx, a.np1)
     if roof is not None:
@@ -535,21 +675,15 @@ def main():
         "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
         "projection": line_proj,
         "fgmres": fgm,
+        "mixed_varying_b": vary,
         "autotuned": {"fp64": tune64, "mixed": tunemx},
         "kernel_timing": "HIP events around every launch of the instrumented replay of the timed solves",
-        "kernels_mixed": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
-                              "launches_per_solve": v["launches"] / a.steps,
-                              "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
-                          for k, v in profmx.items() if v["launches"]},
-        "kernels_fp64": {k: {"ms_per_solve": round(v["ms"] / a.steps, 4),
-                             "launches_per_solve": v["launches"] / a.steps,
-                             "GBs": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
-                         for k, v in prof64.items() if v["launches"]},
+        "kernels_mixed": class_table(profmx, a.steps),
+        "kernels_fp64": class_table(prof64, a.steps),
     }
     s.close()
     if not a.no_hbm:
-        line["roofline_512x512x96"] = hbm_roofline(a.np1, a.tol, max(2, a.steps // 4),
-                                                   not a.no_cpu)
+        line["roofline_cfg3"] = cfg3_leg(a.np1, a.tol, max(4, a.steps // 4), not a.no_cpu)
     if not a.no_cpu:
         cb, xo, mo, orc = cpu_baseline(p, a.tol, a.cpu_seconds)
         line["cpu_baseline"] = cb
