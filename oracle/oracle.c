@@ -48,7 +48,10 @@ struct orc_op {
   orc_grid g;
   long N;
   double *d, *ux, *lx, *uy, *ly, *uz, *lz;
+  const orc_mg *mg; /* PressurePre of the reference (multigrid.c) when set */
 };
+
+void orc_set_pressurepre(orc_op *A, const orc_mg *M) { A->mg = M; }
 
 long orc_n(const orc_op *A) { return A->N * A->g.np1; }
 
@@ -513,7 +516,8 @@ void orc_psolve1(const orc_op *A, double *y, const double *x) {
   const long n = orc_n(A);
   double *t = (double *)malloc(sizeof(double) * n);
   for (long c = 0; c < n; ++c) t[c] = -x[c]; /* b1pBTx2: b%p := -x%p */
-  line_solve_impl(A, y, t, 0);
+  if (A->mg) orc_mg_pressurepre(A->mg, y, t); /* PressurePre(b, b) */
+  else line_solve_impl(A, y, t, 0);
   free(t);
 }
 

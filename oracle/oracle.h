@@ -94,6 +94,46 @@ void orc_update(const orc_grid *g, const double *FU, const double *FV, const dou
 void orc_set_threads(int n);
 int orc_get_threads(void);
 
+/* ---- the reference's own preconditioner, PressurePre (multigrid.c) -------------------
+ * Semicoarsened multigrid with symmetric lexicographic y-plane Gauss-Seidel smoothing,
+ * each plane solved by a 2D multigrid with symmetric x-line Gauss-Seidel smoothing and
+ * exact z-line Thomas solves (MATRIX/PreCond_Mod.f90:599-622 -> VCycleCG,
+ * MULTI/JacAccGrav_Mod.f90:4468-4517; hierarchy :685-1484; cycles MULTI/TriDiagT_Mod.f90,
+ * TriDiagTB_Mod.f90). */
+typedef struct {
+  int ref_level;   /* RefLevel of the block (grid file; 4 in Block.grid), >= 1 */
+  int gauss_iter2; /* Control_Mod GaussIter2 (default 1) */
+  int gauss_iter3; /* Control_Mod GaussIter3 (default 1) */
+  int mult_iter2;  /* Control_Mod MultIter2 (default 1) */
+  int mult_iter3;  /* Control_Mod MultIter3 (default 1; MultiTriT only) */
+} orc_mg_control;
+
+typedef struct orc_mg orc_mg;
+
+void orc_mg_default_control(orc_mg_control *c);
+/* The level hierarchy from the geometry and the thermodynamic weights (the inputs of
+ * orc_assemble_weights); NULL for an invalid control. */
+orc_mg *orc_mg_build(const orc_grid *g, const orc_mg_control *ctl, const double *FU,
+                     const double *FV, const double *FW, const double *VolC, const double *dx,
+                     const double *dy, const double *dz, const double *DTU, const double *DTV,
+                     const double *DTW, const double *DUT);
+void orc_mg_free(orc_mg *M);
+/* x = PressurePre(r): x = 0, then the V-cycles of VCycleCG; r, x in p(np1,nx,ny,nz) order. */
+void orc_mg_pressurepre(const orc_mg *M, double *x, const double *r);
+/* Grid of 3D level `level3` (dims = nx, ny, nz); returns the number of 2D levels of its
+ * planes, -1 for a level that does not exist. */
+int orc_mg_levels(const orc_mg *M, int level3, int *dims);
+/* y = A x with the matrix of 3D level `level3` (MatVecT_3D / MatVecTB_3D). */
+int orc_mg_apply_level(const orc_mg *M, int level3, double *y, const double *x);
+/* Residual restriction from level3 to level3+1 (RestrictT_3D / RestrictTB_3D) and the
+ * prolongation back (ProlongateT_3D / ProlongateTB_3D). */
+int orc_mg_restrict(const orc_mg *M, int level3, double *rc, const double *r);
+int orc_mg_prolong(const orc_mg *M, int level3, double *x, const double *xc);
+/* Use M (NULL: the vertical-line solve) as the preconditioner of A's PSolve1 and hence of
+ * the Krylov solvers above (forward solves only; the transposed solves keep the line
+ * preconditioner). */
+void orc_set_pressurepre(orc_op *A, const orc_mg *M);
+
 #ifdef __cplusplus
 }
 #endif

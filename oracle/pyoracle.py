@@ -23,6 +23,11 @@ class OrcGrid(C.Structure):
                 ("fac_anela", C.c_double)]
 
 
+class MGControl(C.Structure):
+    _fields_ = [("ref_level", C.c_int), ("gauss_iter2", C.c_int), ("gauss_iter3", C.c_int),
+                ("mult_iter2", C.c_int), ("mult_iter3", C.c_int)]
+
+
 def build(quiet: bool = True) -> str:
     subprocess.run(["make", "-C", HERE], check=True,
                    stdout=subprocess.DEVNULL if quiet else None)
@@ -61,6 +66,17 @@ def lib():
         L.orc_update.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 16
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_get_threads.restype = C.c_int
+        L.orc_mg_default_control.argtypes = [C.POINTER(MGControl)]
+        L.orc_mg_build.restype = C.c_void_p
+        L.orc_mg_build.argtypes = [C.POINTER(OrcGrid), C.POINTER(MGControl)] + [_dp] * 11
+        L.orc_mg_free.argtypes = [C.c_void_p]
+        L.orc_mg_pressurepre.argtypes = [C.c_void_p, _dp, _dp]
+        L.orc_mg_levels.restype = C.c_int
+        L.orc_mg_levels.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        for f in ("orc_mg_apply_level", "orc_mg_restrict", "orc_mg_prolong"):
+            getattr(L, f).restype = C.c_int
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_set_pressurepre.argtypes = [C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -257,3 +273,70 @@ def to_csr(orc: Oracle):
     A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
                       shape=(2 * N, 2 * N))
     return A
+
+
+class PressurePre:
+    """The reference's own preconditioner (PressurePre -> VCycleCG: semicoarsened multigrid
+    with symmetric y-plane / x-line Gauss-Seidel smoothing and z-line Thomas solves) built
+    on an Oracle's geometry and weights.  ``attach`` makes it the PSolve1 of that oracle's
+    forward Krylov solves."""
+
+    def __init__(self, orc: Oracle, ref_level=4, gauss_iter2=1, gauss_iter3=1, mult_iter2=1,
+                 mult_iter3=1):
+        L = lib()
+        self.orc = orc
+        self.ctl = MGControl(ref_level, gauss_iter2, gauss_iter3, mult_iter2, mult_iter3)
+        DTU, DTV, DTW, DUT = (_c(a) for a in orc._face_weights())
+        self._w = (DTU, DTV, DTW, DUT)
+        self._h = L.orc_mg_build(C.byref(orc.g), C.byref(self.ctl),
+                                 *[_p(a) for a in orc._keep], _p(DTU), _p(DTV), _p(DTW),
+                                 _p(DUT))
+        if not self._h:
+            raise ValueError("invalid multigrid control")
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                self.detach()
+                lib().orc_mg_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def __call__(self, r):
+        r = _c(r)
+        x = np.empty_like(r)
+        lib().orc_mg_pressurepre(self._h, _p(x), _p(r))
+        return x
+
+    def levels(self, level3):
+        dims = (C.c_int * 3)()
+        nq = lib().orc_mg_levels(self._h, level3, dims)
+        return tuple(dims), nq
+
+    def _lvl(self, fname, level3, x, nout):
+        y = np.empty(nout)
+        rc = getattr(lib(), fname)(self._h, level3, _p(y), _p(_c(x)))
+        if rc:
+            raise ValueError(f"{fname}: no level {level3}")
+        return y
+
+    def size(self, level3):
+        (nx, ny, nz), _ = self.levels(level3)
+        return nx * ny * nz * self.orc.prob.np1
+
+    def apply_level(self, level3, x):
+        return self._lvl("orc_mg_apply_level", level3, x, self.size(level3))
+
+    def restrict(self, level3, r):
+        return self._lvl("orc_mg_restrict", level3, r, self.size(level3 + 1))
+
+    def prolong(self, level3, xc):
+        return self._lvl("orc_mg_prolong", level3, xc, self.size(level3))
+
+    def attach(self):
+        lib().orc_set_pressurepre(self.orc._h, self._h)
+
+    def detach(self):
+        if getattr(self.orc, "_h", None):
+            lib().orc_set_pressurepre(self.orc._h, None)

@@ -1,0 +1,229 @@
+"""Pins of the oracle's restatement of the reference's own preconditioner, PressurePre
+(oracle/multigrid.c): semicoarsened multigrid with symmetric lexicographic y-plane
+Gauss-Seidel smoothing, plane solves by a 2D multigrid with symmetric x-line Gauss-Seidel
+smoothing and exact z-line Thomas solves.
+
+Each check compares with something the restatement does not compute itself:
+- the finest level's matrix is the pinned operator (tests/test_oracle_pins.py);
+- one vertical column reduces every sweep to the Thomas solve of that column;
+- summation restriction is the transpose of piecewise-constant prolongation;
+- on a uniform grid the coarsened level equals the operator assembled directly on the
+  coarse grid (face areas, volumes and spacings of the coarse cells);
+- the symmetric plane sweep (with the reference's extra top-plane solve) and the symmetric
+  line sweep equal block Gauss-Seidel sweeps done with library solves (scipy);
+- as a stationary iteration the preconditioner converges to the direct sparse solve.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from asam_amd import workload as W
+from oracle import pyoracle as P
+
+
+def ridge(nx, ny, nz, np1, grad_full=1, seed=3):
+    return W.make_problem(nx, ny, nz, 1000.0, 1000.0, 250.0, H=600.0, a=2500.0,
+                          ridge_xc=0.5 * nx * 1000.0, np1=np1, grad_full=grad_full, seed=seed)
+
+
+def live(p, v):
+    return np.where(np.repeat(p.VolC.reshape(-1) > 0.0, p.np1), v, 0.0)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = [dict(np1=1, grad_full=1), dict(np1=1, grad_full=0), dict(np1=2, grad_full=1),
+         dict(np1=2, grad_full=0)]
+IDS = ["T-full", "T-dual", "TB-full", "TB-dual"]
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("dims", [(24, 20, 16), (13, 7, 9), (5, 1, 11)])
+def test_finest_level_is_the_operator(case, dims):
+    p = ridge(*dims, **case)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=3)
+    x = live(p, W.normal(p.n, 5))
+    e = rel(M.apply_level(0, x), orc.apply(x))
+    print(f"{dims} {case}: level-0 matrix vs operator {e:.2e}")
+    assert e < 1e-14
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+@pytest.mark.parametrize("ref", [1, 2, 3])
+def test_one_column_is_the_line_solve(np1, ref):
+    """nx = ny = 1: every plane is one line, so the plane and line sweeps are exact Thomas
+    solves and the coarse corrections carry only round-off."""
+    p = ridge(1, 1, 24, np1)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=ref)
+    r = live(p, W.normal(p.n, 7))
+    e = rel(M(r), orc.line_solve(r))
+    print(f"np1={np1} RefLevel {ref}: PressurePre vs Thomas {e:.2e}")
+    assert e < 1e-13
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+@pytest.mark.parametrize("dims", [(24, 20, 16), (13, 7, 9), (5, 3, 11)])
+def test_restriction_is_prolongation_transpose(np1, dims):
+    p = ridge(*dims, np1)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=3)
+    for lvl in (0, 1):
+        r = W.normal(M.size(lvl), 11 + lvl)
+        xc = W.normal(M.size(lvl + 1), 13 + lvl)
+        a = float(np.dot(M.restrict(lvl, r), xc))
+        b = float(np.dot(r, M.prolong(lvl, xc)))
+        print(f"{dims} np1={np1} level {lvl}: <R r, xc> {a:.15e} <r, P xc> {b:.15e}")
+        assert abs(a - b) <= 1e-13 * (abs(a) + abs(b))
+        # piecewise constant: the prolongation of a constant is that constant
+        assert np.all(M.prolong(lvl, np.ones(M.size(lvl + 1))) == 1.0)
+
+
+def uniform_weights(p):
+    g = p.grid
+    return (np.full((g.nz, g.ny, g.nx + 1), 300.0), np.full((g.nz, g.ny + 1, g.nx), 300.0),
+            np.full((g.nz + 1, g.ny, g.nx), 300.0), np.full((g.nz, g.ny, g.nx), 400.0))
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_uniform_coarse_level_is_the_coarse_grid_operator(case):
+    """No terrain, uniform weights: level 1 (all directions coarsened by 2) is the operator
+    assembled on the grid of doubled spacings, whose face areas and volumes are the sums
+    of the fine ones."""
+    nx, ny, nz = 12, 8, 10
+    p = W.make_problem(nx, ny, nz, 1000.0, 1000.0, 250.0, H=0.0, **case)
+    orc = P.Oracle(p, weights=uniform_weights(p))
+    M = P.PressurePre(orc, ref_level=2)
+    pc = W.make_problem(nx // 2, ny // 2, nz // 2, 2000.0, 2000.0, 500.0, H=0.0, **case)
+    pc.beta0_dt, pc.grav = p.beta0_dt, p.grav
+    oc = P.Oracle(pc, weights=uniform_weights(pc))
+    assert M.levels(1)[0] == (nx // 2, ny // 2, nz // 2)
+    x = W.normal(pc.n, 17)
+    e = rel(M.apply_level(1, x), oc.apply(x))
+    print(f"{case}: coarse level vs coarse-grid operator {e:.2e}")
+    assert e < 1e-13
+
+
+def block(A, idx):
+    return A[idx][:, idx].tocsc()
+
+
+def plane_index(p, k):
+    """Unknowns of the y-plane k (all x and z), in the oracle's order."""
+    g = p.grid
+    c = np.array([i + g.nx * (k + g.ny * z) for z in range(g.nz) for i in range(g.nx)])
+    if p.np1 == 2:
+        c = np.stack([2 * c, 2 * c + 1], axis=1).reshape(-1)
+    return c
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_plane_sweep_order(np1):
+    """RefLevel 1 and converged plane solves (MultIter2 = 40): one PressurePre is the
+    single-level V-cycle, two symmetric block Gauss-Seidel sweeps over the y-planes, each
+    forward over all planes, backward from the second-highest plane to the first, then the
+    top plane once more (TriDiagT_Mod.f90:951-1007).  Done here with spsolve per plane."""
+    p = ridge(10, 6, 8, np1)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=1, mult_iter2=40)
+    A = P.to_csr(orc).tocsr()
+    ny = p.grid.ny
+    I = [plane_index(p, k) for k in range(ny)]
+    D = [block(A, I[k]) for k in range(ny)]
+    b = live(p, W.normal(p.n, 19))
+    x = np.zeros(p.n)
+
+    def solve_plane(k):
+        rhs = b[I[k]] - A[I[k]] @ x + D[k] @ x[I[k]]
+        x[I[k]] = spla.spsolve(D[k], rhs)
+
+    for sweep in range(2):
+        for k in range(ny):
+            solve_plane(k)
+        for k in range(ny - 2, 0, -1):
+            solve_plane(k)
+        solve_plane(0)
+        solve_plane(ny - 1)
+    e = rel(M(b), x)
+    print(f"np1={np1}: PressurePre vs two brute-force symmetric plane sweeps {e:.2e}")
+    assert e < 1e-10
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_line_sweep_order(np1):
+    """One y-plane (ny = 1), RefLevel 1, MultIter2 = 1: the 3D level calls the plane solve
+    twice (its down and up smoothing), each a single-level 2D V-cycle of two symmetric
+    x-line Gauss-Seidel sweeps (TriDiagT_Mod.f90:430-474): four sweeps, each forward over
+    all lines, backward from the second-highest line to the second, then the first line.
+    Done here with a library tri-diagonal solve per line."""
+    p = ridge(9, 1, 7, np1)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=1)
+    A = P.to_csr(orc).tocsr()
+    nx, nz = p.grid.nx, p.grid.nz
+    I = []
+    for i in range(nx):
+        c = np.array([i + nx * z for z in range(nz)])
+        I.append(c if np1 == 1 else np.stack([2 * c, 2 * c + 1], axis=1).reshape(-1))
+    D = [block(A, I[i]) for i in range(nx)]
+    b = live(p, W.normal(p.n, 23))
+    x = np.zeros(p.n)
+
+    def solve_line(i):
+        rhs = b[I[i]] - A[I[i]] @ x + D[i] @ x[I[i]]
+        x[I[i]] = spla.spsolve(D[i], rhs)
+
+    for sweep in range(4):
+        for i in range(nx):
+            solve_line(i)
+        for i in range(nx - 2, 0, -1):
+            solve_line(i)
+        solve_line(0)
+    e = rel(M(b), x)
+    print(f"np1={np1}: PressurePre vs four brute-force symmetric line sweeps {e:.2e}")
+    assert e < 1e-12
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("ref", [1, 2, 3])
+def test_stationary_iteration_converges_to_spsolve(case, ref):
+    p = ridge(16, 12, 12, **case)
+    orc = P.Oracle(p)
+    M = P.PressurePre(orc, ref_level=ref)
+    A = P.to_csr(orc)
+    act = np.repeat(p.VolC.reshape(-1) > 0.0, p.np1)
+    xs = np.zeros(p.n)
+    xs[act] = spla.spsolve(A[act][:, act].tocsc(), p.b[act])
+    x = np.zeros(p.n)
+    hist = []
+    for it in range(40):
+        x = x + M(p.b - orc.apply(x))
+        hist.append(rel(live(p, x), xs))
+        if hist[-1] < 1e-12:
+            break
+    print(f"{case} RefLevel {ref}: error after each cycle {['%.1e' % h for h in hist[:6]]}...")
+    assert hist[-1] < 1e-12
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_krylov_with_pressurepre(np1):
+    """FGMRES and BiCGStab with the reference's preconditioner reach the direct solution and
+    take fewer steps than with the vertical-line preconditioner."""
+    p = ridge(24, 20, 16, np1)
+    orc = P.Oracle(p)
+    xl, itl, _ = orc.fgmres(p.b, 20, 1e-10, 10)
+    M = P.PressurePre(orc, ref_level=3)
+    M.attach()
+    xm, itm, rr = orc.fgmres(p.b, 20, 1e-10, 10)
+    xb, mb, _, _ = orc.bicgstab(p.b, 200, 1e-10)
+    M.detach()
+    res = rel(orc.matvec1(xm), p.b)
+    print(f"np1={np1}: FGMRES steps line {itl}, PressurePre {itm}; BiCGStab MaxIter {mb}; "
+          f"true rel residual {res:.1e}")
+    assert res < 2e-10 and rel(orc.matvec1(xb), p.b) < 2e-10
+    assert rel(xm, xl) < 1e-8
+    assert itm < itl
