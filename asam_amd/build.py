@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(LIBDIR, "libasam_sound.so")
 SOURCES = ["setup.hip", "apply.hip", "line.hip", "blas.hip", "solver.hip", "batch.hip",
-           "project.hip", "transport.hip"]
+           "project.hip", "transport.hip", "noalloc.hip"]
 
 
 def hipcc() -> str:

@@ -222,6 +222,20 @@ int asam_dev_dot(asam_sound_ctx *ctx, size_t n, const double *x, const double *y
 /* y = V .* x (component-wise cell volume, the mass matrix of the stage system). */
 int asam_dev_mass(asam_sound_ctx *ctx, double *y, const double *x);
 
+/* Allocation-free form of asam_dev_dot.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code inside HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Needs no context: on the current HIP device it
+ * enqueues sum(x*y) over n doubles on `stream` (a hipStream_t; NULL for the null stream)
+ * and writes the result to the device double *dot, without synchronising; the sum has the
+ * same bits as asam_dev_dot's.  Its partial sums live in a static device array of the
+ * library (1024 doubles per device, held until the process ends); nothing is allocated.
+ * Calls that overlap on one device (other streams or host threads) share that array and
+ * race.  n at most ASAM_DOT_NOALLOC_MAX_N; returns ASAM_E_INVALID for larger n or a null
+ * pointer, ASAM_E_HIP if the launch fails. */
+#define ASAM_DOT_NOALLOC_MAX_N 4294967296LL /* 2^32, the longest vector a context holds */
+int asam_dev_dot_noalloc(void *stream, size_t n, const double *x, const double *y,
+                         double *dot);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Average device time (ms, HIP events on the context's stream) of one launch of the
  * stencil apply (which=0), line preconditioner (which=1) or fused BiCGStab update
