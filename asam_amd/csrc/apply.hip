@@ -17,12 +17,33 @@ struct ApplyShape {
   int gx, gy, kp, ntiles, nblocks;
 };
 
-// Tiles of 64x4 columns by kp planes, at most ~4096 workgroups, padded to a multiple of
-// the 8 XCDs so that the physical -> logical remap below is a bijection.
+// Rows per band of the band-ordered variant: the tuning's value, or ny/8 rounded up so that
+// each XCD owns one band.
+static int band_rows(const Geom& g) {
+  const int byw = 256 / g.tune.apply_bx;
+  int rb = g.tune.apply_band_rows > 0 ? g.tune.apply_band_rows : (g.ny + 7) / 8;
+  if (rb < byw) rb = byw;
+  return rb;
+}
+
+// Variants 0 and 1: tiles of bx x (256/bx) columns by kp planes, at most ~apply_wg
+// workgroups, padded to a multiple of the 8 XCDs so that the physical -> logical remap below
+// is a bijection.  Variant 2: apply_wg workgroups (a multiple of 8) looping over the tiles
+// of their XCD's bands.
 static ApplyShape apply_shape(const Geom& g) {
   ApplyShape a;
   const int bxw = g.tune.apply_bx, byw = 256 / bxw;
   a.gx = (g.nx + bxw - 1) / bxw;
+  if (g.tune.apply_variant == 2) {
+    const int rb = band_rows(g);
+    a.gy = (rb + byw - 1) / byw;  // row tiles per band
+    a.kp = 1;
+    a.ntiles = 0;
+    int nb = (g.tune.apply_wg / 8) * 8;
+    if (nb < 8) nb = 8;
+    a.nblocks = nb;
+    return a;
+  }
   a.gy = (g.ny + byw - 1) / byw;
   long cols = (long)a.gx * a.gy;
   long want = g.tune.apply_wg;
@@ -38,18 +59,14 @@ static ApplyShape apply_shape(const Geom& g) {
 
 int apply_blocks(const Geom& g) { return apply_shape(g).nblocks; }
 
+// The operator at one cell (i, j, k) with the fused options (k_apply and k_apply_band).
+// Neighbour indices are clamped to the cell itself at walls: the face difference
+// (q_c - q_n) is then exactly zero, so no branch separates the loads.
 template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typename VU>
-__global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
-  if (a.done && *a.done) return;
+__device__ __forceinline__ void apply_cell(const KApply<CT, VX, VO, VU>& a, int i, int j, int k,
+                                           double (&acc)[2]) {
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long sy = nx, sz = (long)nx * ny;
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
-  // range of tiles (consecutive z-planes of the same columns) so that z- and
-  // y-neighbour re-reads hit that XCD's L2.
-  const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
-  const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
-  double acc[2] = {0.0, 0.0};
   const CT* __restrict__ kx = a.c.kx;
   const CT* __restrict__ ky = a.c.ky;
   const CT* __restrict__ kz = a.c.kz;
@@ -62,137 +79,194 @@ __global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
   const VX* __restrict__ X = a.x;
   const float* __restrict__ DX = a.dx;
   const CT fa = a.fa;
+  const long c = i + sy * j + sz * k;
+  if constexpr ((FL & AF_X0) != 0) {
+    // x = 0: every stencil term is a signed zero, so b + s equals b; same accumulation
+    // order as the full form
+    static_assert((FL & AF_B) != 0 && (FL & (AF_DX | AF_YU)) == 0, "x = 0 residual form");
+    const bool dead = a.dead[c] != 0;
+    if constexpr (NP == 1) {
+      const double yv = dead ? 0.0 : a.b[c];
+      a.y[c] = (VO)yv;
+      if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv;
+      if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
+    } else {
+      const double2 b0 = *reinterpret_cast<const double2*>(a.b + 2 * c);
+      const double yv1 = dead ? 0.0 : b0.x, yv2 = dead ? 0.0 : b0.y;
+      st2(a.y, c, yv1, yv2);
+      if constexpr ((FL & AF_Y2) != 0) st2(a.y2, c, yv1, yv2);
+      if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+    }
+    return;
+  }
+  const long mW = i > 0 ? c - 1 : c, mE = i + 1 < nx ? c + 1 : c;
+  const long mS = j > 0 ? c - sy : c, mN = j + 1 < ny ? c + sy : c;
+  const long mD = k > 0 ? c - sz : c, mU = k + 1 < nz ? c + sz : c;
+  double yv1, yv2 = 0.0;
+  if constexpr (NP == 1) {
+    auto LX = [&](long m) -> CT {
+      CT v = (CT)X[m];
+      if constexpr ((FL & AF_DX) != 0) v += (CT)DX[m];
+      return v;
+    };
+    const CT x0 = LX(c), d0 = dut[c];
+    CT s = fa * vol[c] * x0;
+    if constexpr (TR == 0) {
+      const CT q0 = d0 * x0;
+      s += kx[mW] * (q0 - dut[mW] * LX(mW));
+      s += kx[c] * (q0 - dut[mE] * LX(mE));
+      s += ky[mS] * (q0 - dut[mS] * LX(mS));
+      s += ky[c] * (q0 - dut[mN] * LX(mN));
+      s += kz[mD] * (q0 - dut[mD] * LX(mD));
+      s += kz[c] * (q0 - dut[mU] * LX(mU));
+    } else {
+      CT h = kx[mW] * (x0 - LX(mW));
+      h += kx[c] * (x0 - LX(mE));
+      h += ky[mS] * (x0 - LX(mS));
+      h += ky[c] * (x0 - LX(mN));
+      h += kz[mD] * (x0 - LX(mD));
+      h += kz[c] * (x0 - LX(mU));
+      s += d0 * h;
+    }
+    // Op = -A x; residual form y = b - Op(x) = b + A x, zero on dead cells
+    if constexpr ((FL & AF_B) != 0) {
+      const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
+                                           kz[mD] + kz[c] == (CT)0;
+      yv1 = dead ? 0.0 : a.b[c] + (double)s;
+    } else {
+      yv1 = -(double)s;
+    }
+    if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
+    a.y[c] = (VO)yv1;
+    if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv1;
+    if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1;
+    if constexpr ((FL & AF_YU) != 0) acc[1] += yv1 * (double)ld_once(a.u, c);
+  } else {
+    auto LX = [&](long m) -> V2<CT> {
+      V2<VX> v = ld2(X, m);
+      V2<CT> r{(CT)v.a, (CT)v.b};
+      if constexpr ((FL & AF_DX) != 0) {
+        V2<float> e = ld2(DX, m);
+        r.a += (CT)e.a; r.b += (CT)e.b;
+      }
+      return r;
+    };
+    const V2<CT> x0 = LX(c);
+    const CT d0 = dut[c], v0 = vol[c];
+    const V2<CT> xW = LX(mW), xE = LX(mE), xS = LX(mS), xN = LX(mN);
+    const V2<CT> xD = LX(mD), xU = LX(mU);
+    // gravity couplings of the z-faces below and above (zero at the walls)
+    const CT gD = k > 0 ? gzz[mD] : (CT)0, gU = gzz[c];
+    CT s1 = fa * v0 * x0.a, s2 = v0 * x0.b;
+    if constexpr (TR == 0) {
+      const CT q0 = d0 * x0.a;
+      auto hf = [&](CT kk, CT th, CT dn, CT xn) {
+        const CT t = kk * (q0 - dn * xn);
+        s1 += th * t; s2 += t;
+      };
+      hf(kx[mW], tx[mW], dut[mW], xW.a);
+      hf(kx[c], tx[c], dut[mE], xE.a);
+      hf(ky[mS], ty[mS], dut[mS], xS.a);
+      hf(ky[c], ty[c], dut[mN], xN.a);
+      {
+        const CT t = kz[mD] * (q0 - dut[mD] * xD.a);
+        const CT B = gD * (vol[mD] * xD.b + v0 * x0.b);
+        s1 += tz[mD] * (t + B); s2 += t + B;
+      }
+      {
+        const CT t = kz[c] * (q0 - dut[mU] * xU.a);
+        const CT B = gU * (v0 * x0.b + vol[mU] * xU.b);
+        s1 += tz[c] * (t - B); s2 += t - B;
+      }
+    } else {
+      CT h = 0;
+      auto hf = [&](CT kk, CT th, V2<CT> xn) {
+        h += kk * ((th * x0.a + x0.b) - (th * xn.a + xn.b));
+      };
+      hf(kx[mW], tx[mW], xW);
+      hf(kx[c], tx[c], xE);
+      hf(ky[mS], ty[mS], xS);
+      hf(ky[c], ty[c], xN);
+      {
+        const CT th = tz[mD];
+        const CT zc = th * x0.a + x0.b, zn = th * xD.a + xD.b;
+        h += kz[mD] * (zc - zn);
+        s2 += gD * v0 * (zc - zn);
+      }
+      {
+        const CT th = tz[c];
+        const CT zc = th * x0.a + x0.b, zn = th * xU.a + xU.b;
+        h += kz[c] * (zc - zn);
+        s2 += gU * v0 * (zn - zc);
+      }
+      s1 += d0 * h;
+    }
+    if constexpr ((FL & AF_B) != 0) {
+      const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
+      const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
+                                           kz[mD] + kz[c] == (CT)0;
+      yv1 = dead ? 0.0 : bb.x + (double)s1;
+      yv2 = dead ? 0.0 : bb.y + (double)s2;
+    } else {
+      yv1 = -(double)s1; yv2 = -(double)s2;
+    }
+    if constexpr ((FL & AF_DX) != 0) st2(a.xw, c, (double)x0.a, (double)x0.b);
+    st2(a.y, c, yv1, yv2);
+    if constexpr ((FL & AF_Y2) != 0) st2(a.y2, c, yv1, yv2);
+    if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+    if constexpr ((FL & AF_YU) != 0) {
+      const V2<VU> uu = ld2(a.u, c);
+      acc[1] += yv1 * (double)uu.a + yv2 * (double)uu.b;
+    }
+  }
+}
+
+template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typename VU>
+__global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
+  if (a.done && *a.done) return;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous
+  // range of tiles (consecutive z-planes of the same columns) so that z- and
+  // y-neighbour re-reads hit that XCD's L2.
+  const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
+  const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
+  double acc[2] = {0.0, 0.0};
   if (L < a.ntiles) {
     const int bx = L % a.gx, rest = L / a.gx, by = rest % a.gy, kt = rest / a.gy;
     const int i = bx * a.bxw + threadIdx.x, j = by * a.byw + threadIdx.y;
-    const int k0 = kt * a.kp, k1 = min(nz, k0 + a.kp);
-    if (i < nx && j < ny) {
+    const int k0 = kt * a.kp, k1 = min(a.nz, k0 + a.kp);
+    if (i < a.nx && j < a.ny) {
 #pragma unroll 2
-      for (int k = k0; k < k1; ++k) {
-        const long c = i + sy * j + sz * k;
-        // Neighbour indices clamped to the cell itself at walls: the face difference
-        // (q_c - q_n) is then exactly zero, so no branch separates the loads.
-        const long mW = i > 0 ? c - 1 : c, mE = i + 1 < nx ? c + 1 : c;
-        const long mS = j > 0 ? c - sy : c, mN = j + 1 < ny ? c + sy : c;
-        const long mD = k > 0 ? c - sz : c, mU = k + 1 < nz ? c + sz : c;
-        double yv1, yv2 = 0.0;
-        if constexpr (NP == 1) {
-          auto LX = [&](long m) -> CT {
-            CT v = (CT)X[m];
-            if constexpr ((FL & AF_DX) != 0) v += (CT)DX[m];
-            return v;
-          };
-          const CT x0 = LX(c), d0 = dut[c];
-          CT s = fa * vol[c] * x0;
-          if constexpr (TR == 0) {
-            const CT q0 = d0 * x0;
-            s += kx[mW] * (q0 - dut[mW] * LX(mW));
-            s += kx[c] * (q0 - dut[mE] * LX(mE));
-            s += ky[mS] * (q0 - dut[mS] * LX(mS));
-            s += ky[c] * (q0 - dut[mN] * LX(mN));
-            s += kz[mD] * (q0 - dut[mD] * LX(mD));
-            s += kz[c] * (q0 - dut[mU] * LX(mU));
-          } else {
-            CT h = kx[mW] * (x0 - LX(mW));
-            h += kx[c] * (x0 - LX(mE));
-            h += ky[mS] * (x0 - LX(mS));
-            h += ky[c] * (x0 - LX(mN));
-            h += kz[mD] * (x0 - LX(mD));
-            h += kz[c] * (x0 - LX(mU));
-            s += d0 * h;
-          }
-          // Op = -A x; residual form y = b - Op(x) = b + A x, zero on dead cells
-          if constexpr ((FL & AF_B) != 0) {
-            const bool dead = vol[c] == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
-                                                 kz[mD] + kz[c] == (CT)0;
-            yv1 = dead ? 0.0 : a.b[c] + (double)s;
-          } else {
-            yv1 = -(double)s;
-          }
-          if constexpr ((FL & AF_DX) != 0) a.xw[c] = (double)x0;
-          a.y[c] = (VO)yv1;
-          if constexpr ((FL & AF_Y2) != 0) a.y2[c] = (VO)yv1;
-          if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1;
-          if constexpr ((FL & AF_YU) != 0) acc[1] += yv1 * (double)a.u[c];
-        } else {
-          auto LX = [&](long m) -> V2<CT> {
-            V2<VX> v = ld2(X, m);
-            V2<CT> r{(CT)v.a, (CT)v.b};
-            if constexpr ((FL & AF_DX) != 0) {
-              V2<float> e = ld2(DX, m);
-              r.a += (CT)e.a; r.b += (CT)e.b;
-            }
-            return r;
-          };
-          const V2<CT> x0 = LX(c);
-          const CT d0 = dut[c], v0 = vol[c];
-          const V2<CT> xW = LX(mW), xE = LX(mE), xS = LX(mS), xN = LX(mN);
-          const V2<CT> xD = LX(mD), xU = LX(mU);
-          // gravity couplings of the z-faces below and above (zero at the walls)
-          const CT gD = k > 0 ? gzz[mD] : (CT)0, gU = gzz[c];
-          CT s1 = fa * v0 * x0.a, s2 = v0 * x0.b;
-          if constexpr (TR == 0) {
-            const CT q0 = d0 * x0.a;
-            auto hf = [&](CT kk, CT th, CT dn, CT xn) {
-              const CT t = kk * (q0 - dn * xn);
-              s1 += th * t; s2 += t;
-            };
-            hf(kx[mW], tx[mW], dut[mW], xW.a);
-            hf(kx[c], tx[c], dut[mE], xE.a);
-            hf(ky[mS], ty[mS], dut[mS], xS.a);
-            hf(ky[c], ty[c], dut[mN], xN.a);
-            {
-              const CT t = kz[mD] * (q0 - dut[mD] * xD.a);
-              const CT B = gD * (vol[mD] * xD.b + v0 * x0.b);
-              s1 += tz[mD] * (t + B); s2 += t + B;
-            }
-            {
-              const CT t = kz[c] * (q0 - dut[mU] * xU.a);
-              const CT B = gU * (v0 * x0.b + vol[mU] * xU.b);
-              s1 += tz[c] * (t - B); s2 += t - B;
-            }
-          } else {
-            CT h = 0;
-            auto hf = [&](CT kk, CT th, V2<CT> xn) {
-              h += kk * ((th * x0.a + x0.b) - (th * xn.a + xn.b));
-            };
-            hf(kx[mW], tx[mW], xW);
-            hf(kx[c], tx[c], xE);
-            hf(ky[mS], ty[mS], xS);
-            hf(ky[c], ty[c], xN);
-            {
-              const CT th = tz[mD];
-              const CT zc = th * x0.a + x0.b, zn = th * xD.a + xD.b;
-              h += kz[mD] * (zc - zn);
-              s2 += gD * v0 * (zc - zn);
-            }
-            {
-              const CT th = tz[c];
-              const CT zc = th * x0.a + x0.b, zn = th * xU.a + xU.b;
-              h += kz[c] * (zc - zn);
-              s2 += gU * v0 * (zn - zc);
-            }
-            s1 += d0 * h;
-          }
-          if constexpr ((FL & AF_B) != 0) {
-            const double2 bb = *reinterpret_cast<const double2*>(a.b + 2 * c);
-            const bool dead = v0 == (CT)0 && kx[mW] + kx[c] + ky[mS] + ky[c] +
-                                                 kz[mD] + kz[c] == (CT)0;
-            yv1 = dead ? 0.0 : bb.x + (double)s1;
-            yv2 = dead ? 0.0 : bb.y + (double)s2;
-          } else {
-            yv1 = -(double)s1; yv2 = -(double)s2;
-          }
-          if constexpr ((FL & AF_DX) != 0) st2(a.xw, c, (double)x0.a, (double)x0.b);
-          st2(a.y, c, yv1, yv2);
-          if constexpr ((FL & AF_Y2) != 0) st2(a.y2, c, yv1, yv2);
-          if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
-          if constexpr ((FL & AF_YU) != 0) {
-            const V2<VU> uu = ld2(a.u, c);
-            acc[1] += yv1 * (double)uu.a + yv2 * (double)uu.b;
-          }
-        }
-      }
+      for (int k = k0; k < k1; ++k) apply_cell<NP, TR, FL, CT, VX, VO, VU>(a, i, j, k, acc);
+    }
+  }
+  if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
+    block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
+  }
+}
+
+// Band-ordered flat variant: one cell per thread.  XCD x owns the bands x, x+8, x+16, ...
+// of band_rows rows; its workgroups walk one band at a time, plane by plane, tile after tile
+// (bx x by cells of one plane, x fastest), each workgroup taking every G-th tile of the band.
+// The G workgroups of an XCD so work on a window of consecutive tiles: the rows above and
+// below and the plane below were fetched by the same XCD moments earlier and hit its L2, and
+// HBM sees every array streamed once in address order per band.
+template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typename VU>
+__global__ __launch_bounds__(256) void k_apply_band(KApply<CT, VX, VO, VU> a) {
+  if (a.done && *a.done) return;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int bphys = blockIdx.x, xcd = bphys & 7, slot = bphys >> 3, G = gridDim.x >> 3;
+  const int rb = a.band_rows, nbands = (a.ny + rb - 1) / rb;
+  const int tpp = a.gx * a.gy;            // tiles per plane of a band
+  const long tpb = (long)tpp * a.nz;      // tiles per band
+  double acc[2] = {0.0, 0.0};
+  for (int band = xcd; band < nbands; band += 8) {
+    const int j0 = band * rb, j1 = min(a.ny, j0 + rb);
+    for (long t = slot; t < tpb; t += G) {
+      const int k = (int)(t / tpp), r = (int)(t - (long)k * tpp);
+      const int ty = r / a.gx, tx = r - ty * a.gx;
+      const int i = tx * a.bxw + threadIdx.x, j = j0 + ty * a.byw + threadIdx.y;
+      if (i < a.nx && j < j1) apply_cell<NP, TR, FL, CT, VX, VO, VU>(a, i, j, k, acc);
     }
   }
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
@@ -224,7 +298,12 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
   a.x = (const VX*)r.x; a.dx = r.dx; a.xw = r.xw; a.b = r.b; a.y = (VO*)r.y;
   a.y2 = (VO*)r.y2; a.u = (const VU*)r.u; a.partial = r.partial; a.done = r.done;
   a.dead = r.x0_dead;
-  if constexpr ((FL & AF_X0) != 0) {
+  a.band_rows = band_rows(g);
+  if (g.tune.apply_variant == 2) {
+    // the x = 0 form too, so that it writes the partial sums of the full form
+    hipLaunchKernelGGL((k_apply_band<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
+                       dim3(a.bxw, a.byw), 0, s, a);
+  } else if constexpr ((FL & AF_X0) != 0) {
     // same grid and partial-sum slots in either variant
     hipLaunchKernelGGL((k_apply_march<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
                        dim3(a.bxw, a.byw), 0, s, a);

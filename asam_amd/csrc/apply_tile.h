@@ -29,6 +29,7 @@ struct KApply {
   int nx, ny, nz;
   int gx, gy, kp, ntiles;  // tiles: bx x by columns x kp planes, ntiles in logical order
   int bxw, byw, xcd;
+  int band_rows;           // band-ordered variant: rows per band (gy = row tiles per band)
   CT fa;
   const VX* x;
   const float* dx;

@@ -19,6 +19,9 @@ namespace asam {
 constexpr double kEps = 1.0e-30;          // Kind_Mod Eps
 constexpr double kRd = 1004.0 - 717.0;    // Parameter_Mod Rd = Cpd - Cvd
 constexpr double kCpd = 1004.0;
+constexpr double kRv = 461.0;             // Parameter_Mod Rv, Cpv, Cpl (vapour, liquid water)
+constexpr double kCpv = 1885.0;
+constexpr double kCpl = 4186.0;
 constexpr int kMaxM = 32;                 // largest (F)GMRES restart length
 constexpr int kRedBlocks = 16384;         // partial-sum slots per reduction quantity
 
@@ -32,7 +35,8 @@ struct Coef {
 
 // Launch-shape knobs (set by the autotuner or asam_sound_set_tuning).
 struct Tuning {
-  int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread
+  int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread,
+                          // 2: one cell per thread in band order (each XCD its own bands)
   int apply_wg = 4096;    // target workgroups per apply launch
   int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
   int line_lds = 1;       // LDS staging budget of the line solve's forward sweep per
@@ -40,6 +44,7 @@ struct Tuning {
   int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
   int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
+  int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
 };
 
 struct Geom {
@@ -90,7 +95,8 @@ struct KState {
 
 // ------------------------------------------------------------------ set-up
 void launch_weights(hipStream_t s, const Geom& g, const double* rho, const double* rth,
-                    const double* T, double* theta, double* dut);
+                    const double* T, const double* rhov, const double* rhol,
+                    const double* rhor, double* theta, double* dut);
 void launch_face_theta(hipStream_t s, const Geom& g, const double* vol, const double* theta,
                        double* thu, double* thv, double* thw);
 // Face coefficients from areas, volumes, spacings and face theta (full face shapes).

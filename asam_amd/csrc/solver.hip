@@ -130,12 +130,14 @@ extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning*
   t->apply_xcd = c->g.tune.apply_xcd;
   t->apply_bx = c->g.tune.apply_bx;
   t->line_tiled = c->g.tune.line_tiled;
+  t->apply_band_rows = c->g.tune.apply_band_rows;
   return 0;
 }
 
 extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
-  if (t->apply_variant < 0 || t->apply_variant > 1) return fail(c, ASAM_E_INVALID, "apply_variant");
+  if (t->apply_variant < 0 || t->apply_variant > 2) return fail(c, ASAM_E_INVALID, "apply_variant");
+  if (t->apply_band_rows < 0) return fail(c, ASAM_E_INVALID, "apply_band_rows");
   if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
   if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
     return fail(c, ASAM_E_INVALID, "line_window");
@@ -151,6 +153,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.apply_xcd = t->apply_xcd ? 1 : 0;
   nt.apply_bx = t->apply_bx;
   nt.line_tiled = t->line_tiled ? 1 : 0;
+  nt.apply_band_rows = t->apply_band_rows;
   Geom tg = c->g;
   tg.tune = nt;
   if (apply_blocks(tg) > kRedBlocks) return fail(c, ASAM_E_INVALID, "apply_wg too large");
@@ -270,6 +273,7 @@ extern "C" int asam_sound_set_control(asam_sound_ctx* c, const asam_sound_contro
 static int setup_common(asam_sound_ctx* c, const double* FU, const double* FV, const double* FW,
                         const double* V, const double* dx, const double* dy, const double* dz,
                         const double* rho, const double* rth, const double* T,
+                        const double* rhov, const double* rhol, const double* rhor,
                         const double* DTU, const double* DTV, const double* DTW,
                         const double* DUT) {
   const Geom& g = c->g;
@@ -296,17 +300,26 @@ static int setup_common(asam_sound_ctx* c, const double* FU, const double* FV, c
   double* thv = thu + nfu;
   double* thw = thv + nfv;
   if (rho) {
-    double *drho, *drth, *dT;
+    double *drho, *drth, *dT, *drv = nullptr, *drl = nullptr, *drr = nullptr;
     rc |= dalloc(c, &drho, N); rc |= dalloc(c, &drth, N); rc |= dalloc(c, &dT, N);
     rc |= dalloc(c, &tmp, N);
+    if (rhov) rc |= dalloc(c, &drv, N);
+    if (rhol) rc |= dalloc(c, &drl, N);
+    if (rhor) rc |= dalloc(c, &drr, N);
     if (rc) return ASAM_E_NOMEM;
     HIPCHK(c, hipMemcpyAsync(drho, rho, N * 8, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(drth, rth, N * 8, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(dT, T, N * 8, hipMemcpyHostToDevice, s));
-    launch_weights(s, g, drho, drth, dT, tmp, c->c64.dut);
+    if (drv) HIPCHK(c, hipMemcpyAsync(drv, rhov, N * 8, hipMemcpyHostToDevice, s));
+    if (drl) HIPCHK(c, hipMemcpyAsync(drl, rhol, N * 8, hipMemcpyHostToDevice, s));
+    if (drr) HIPCHK(c, hipMemcpyAsync(drr, rhor, N * 8, hipMemcpyHostToDevice, s));
+    launch_weights(s, g, drho, drth, dT, drv, drl, drr, tmp, c->c64.dut);
     launch_face_theta(s, g, dV, tmp, thu, thv, thw);
     HIPCHK(c, hipStreamSynchronize(s));
     dfree(c, drho); dfree(c, drth); dfree(c, dT); dfree(c, tmp);
+    if (drv) dfree(c, drv);
+    if (drl) dfree(c, drl);
+    if (drr) dfree(c, drr);
   } else {
     HIPCHK(c, hipMemcpyAsync(thu, DTU, nfu * 8, hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemcpyAsync(thv, DTV, nfv * 8, hipMemcpyHostToDevice, s));
@@ -338,7 +351,20 @@ extern "C" int asam_sound_setup(asam_sound_ctx* c, const double* FU, const doubl
     return fail(c, ASAM_E_INVALID, "null input");
   HIPCHK(c, hipSetDevice(c->device));
   return setup_common(c, FU, FV, FW, V, dx, dy, dz, rho, rth, T, nullptr, nullptr, nullptr,
-                      nullptr);
+                      nullptr, nullptr, nullptr, nullptr);
+}
+
+extern "C" int asam_sound_setup_moist(asam_sound_ctx* c, const double* FU, const double* FV,
+                                      const double* FW, const double* V, const double* dx,
+                                      const double* dy, const double* dz, const double* rho,
+                                      const double* rth, const double* T, const double* rhov,
+                                      const double* rhol, const double* rhor) {
+  if (!c) return ASAM_E_INVALID;
+  if (!FU || !FV || !FW || !V || !dx || !dy || !dz || !rho || !rth || !T)
+    return fail(c, ASAM_E_INVALID, "null input");
+  HIPCHK(c, hipSetDevice(c->device));
+  return setup_common(c, FU, FV, FW, V, dx, dy, dz, rho, rth, T, rhov, rhol, rhor, nullptr,
+                      nullptr, nullptr, nullptr);
 }
 
 extern "C" int asam_sound_setup_weights(asam_sound_ctx* c, const double* FU, const double* FV,
@@ -350,8 +376,8 @@ extern "C" int asam_sound_setup_weights(asam_sound_ctx* c, const double* FU, con
   if (!FU || !FV || !FW || !V || !dx || !dy || !dz || !DTU || !DTV || !DTW || !DUT)
     return fail(c, ASAM_E_INVALID, "null input");
   HIPCHK(c, hipSetDevice(c->device));
-  return setup_common(c, FU, FV, FW, V, dx, dy, dz, nullptr, nullptr, nullptr, DTU, DTV, DTW,
-                      DUT);
+  return setup_common(c, FU, FV, FW, V, dx, dy, dz, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, DTU, DTV, DTW, DUT);
 }
 
 // ------------------------------------------------------------------ enqueue helpers
@@ -1359,7 +1385,20 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
           if (rc) { c->g.tune = t0; return rc; }
           if (m < best_ms) { best_ms = m; bt = c->g.tune; }
         }
-  const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48};
+  // band-ordered variant: width, rows per band (0: one band per XCD), workgroups
+  const int rows[] = {0, 16, 32, 64}, bwgs[] = {1024, 2048, 4096};
+  for (int bx : bxs)
+    for (int rb : rows)
+      for (int wg : bwgs) {
+        c->g.tune = bt;
+        c->g.tune.apply_variant = 2; c->g.tune.apply_bx = bx;
+        c->g.tune.apply_band_rows = rb; c->g.tune.apply_wg = wg;
+        double m = 0;
+        int rc = measure(&m);
+        if (rc) { c->g.tune = t0; return rc; }
+        if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+      }
+  const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48, 80};
   const Tuning ba = bt;
   for (int w : windows)
     for (int l : ldss) {

@@ -48,7 +48,7 @@ class Info(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("apply_variant", C.c_int), ("apply_wg", C.c_int), ("line_window", C.c_int),
                 ("line_lds", C.c_int), ("fused_tail", C.c_int), ("apply_xcd", C.c_int),
-                ("apply_bx", C.c_int), ("line_tiled", C.c_int)]
+                ("apply_bx", C.c_int), ("line_tiled", C.c_int), ("apply_band_rows", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -93,6 +93,7 @@ def load(path: str = LIB_PATH):
         "asam_sound_last_error": (C.c_char_p, [_vp]),
         "asam_sound_setup": (i, [_vp] + [dp] * 10),
         "asam_sound_setup_weights": (i, [_vp] + [dp] * 11),
+        "asam_sound_setup_moist": (i, [_vp] + [dp] * 13),
         "asam_sound_solve": (i, [_vp, dp, dp, ip, dp, C.POINTER(Info)]),
         "asam_sound_solve_device": (i, [_vp, _vp, _vp, ip, dp, C.POINTER(Info)]),
         "asam_sound_solve_T": (i, [_vp, dp, dp, ip, dp, C.POINTER(Info)]),
@@ -226,6 +227,14 @@ class SoundSolver:
     def setup(self, FU, FV, FW, VolC, dx, dy, dz, rho, rhotheta, T):
         arrs = [_c(a) for a in (FU, FV, FW, VolC, dx, dy, dz, rho, rhotheta, T)]
         self._chk(self.L.asam_sound_setup(self.h, *[_p(a) for a in arrs]))
+
+    def setup_moist(self, FU, FV, FW, VolC, dx, dy, dz, rho, rhotheta, T, rhov=None, rhol=None,
+                    rhor=None):
+        """JacAccGrav with moisture: vapour, cloud water and rain densities (None: none)."""
+        arrs = [_c(a) for a in (FU, FV, FW, VolC, dx, dy, dz, rho, rhotheta, T)]
+        q = [None if a is None else _c(a) for a in (rhov, rhol, rhor)]
+        self._chk(self.L.asam_sound_setup_moist(self.h, *[_p(a) for a in arrs],
+                                                *[_p(a) if a is not None else None for a in q]))
 
     def setup_weights(self, FU, FV, FW, VolC, dx, dy, dz, DTU, DTV, DTW, DUT):
         arrs = [_c(a) for a in (FU, FV, FW, VolC, dx, dy, dz, DTU, DTV, DTW, DUT)]

@@ -91,8 +91,11 @@ typedef struct {
 /* Launch-shape knobs of the kernels (no effect on results beyond round-off order of
  * the reductions; the defaults are tuned for MI355X). */
 typedef struct {
-  int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread */
-  int apply_wg;      /* target workgroups per stencil-apply launch (256..16384) */
+  int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread,
+                        2: one cell per thread in band order (each XCD streams its own bands
+                        of rows plane by plane) */
+  int apply_wg;      /* target workgroups per stencil-apply launch (256..16384; variant 2:
+                        the workgroups of the launch, each looping over its XCD's tiles) */
   int line_window;   /* levels per load window of the line solve: 1, 4 or 8 */
   int line_lds;      /* LDS budget per workgroup for the line solve's forward sweep: 0 none
                         (staged in the output vector), 1 default (60 KiB), 2..160 that
@@ -102,6 +105,7 @@ typedef struct {
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
   int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
   int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
+  int apply_band_rows; /* variant 2: rows per band (0: ny/8, one band per XCD) */
 } asam_sound_tuning;
 
 /* ---- context ---------------------------------------------------------- */
@@ -131,6 +135,16 @@ const char *asam_sound_last_error(const asam_sound_ctx *ctx);
 int asam_sound_setup(asam_sound_ctx *ctx, const double *FU, const double *FV, const double *FW,
                      const double *VolC, const double *dx, const double *dy, const double *dz,
                      const double *rho, const double *rhotheta, const double *T);
+/* Moist form of asam_sound_setup: vapour rhov, cloud water rhol and rain rhor densities
+ * (nx*ny*nz each; any may be NULL, meaning none of that species) enter DUT as in
+ * WeightDiagCompute 'Density' (JacAccGrav_Mod.f90:4254-4270): RhoD = Rho - RhoV - RhoL - RhoR,
+ * Rm = Rd + Rv RhoV/RhoD, Cpml = Cpd + Cpv RhoV/RhoD + Cpl (RhoL + RhoR)/RhoD,
+ * DUT = T/(rho*theta) (RhoD Rd + RhoV Rv)/(1 - Rm/Cpml).  rho is the total density. */
+int asam_sound_setup_moist(asam_sound_ctx *ctx, const double *FU, const double *FV,
+                           const double *FW, const double *VolC, const double *dx,
+                           const double *dy, const double *dz, const double *rho,
+                           const double *rhotheta, const double *T, const double *rhov,
+                           const double *rhol, const double *rhor);
 /* From geometry and precomputed weights DTU/DTV/DTW (face theta, face shapes) and DUT
  * (cells) -- the inputs of LaplTCompute (JacAccGrav_Mod.f90:893-1055). */
 int asam_sound_setup_weights(asam_sound_ctx *ctx, const double *FU, const double *FV,
