@@ -896,7 +896,7 @@ extern "C" int asam_ens_setup(asam_ens_ctx* c, int m, const double* FU, const do
   Coef<double> k64 = member_coef(c->c64, M, c->np1);
   Coef<float> k32 = member_coef(c->c32, M, c->np1);
   const asam_sound_grid& q = c->grids[m];
-  launch_weights(s, g, drho, drth, dT, tmp, k64.dut);
+  launch_weights(s, g, drho, drth, dT, nullptr, nullptr, nullptr, tmp, k64.dut);
   launch_face_theta(s, g, dV, tmp, th, th + nfu, th + nfu + nfv);
   launch_coef(s, g, q.grad_full, q.beta0_dt, q.grav, dFU, dFV, dFW, dV, ddx, ddy, ddz, th,
               th + nfu, th + nfu + nfv, k64);

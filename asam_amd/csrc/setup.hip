@@ -6,22 +6,34 @@
 namespace asam {
 
 // ===================================================================== set-up
+// WeightDiagCompute 'Density' (JacAccGrav_Mod.f90:4254-4270): theta = rho*theta / rho, and
+// DUT = dp/d(rho*theta) of moist air with vapour rhov and condensate rhol + rhor (each
+// pointer may be null: none of it):
+//   RhoD = Rho - RhoV - RhoL + Eps, Rm = Rd + Rv RhoV/RhoD,
+//   Cpml = Cpd + Cpv RhoV/RhoD + Cpl RhoL/RhoD, Kappa = Rm/Cpml,
+//   DUT = T/(rho*theta + Eps) (RhoD Rd + RhoV Rv)/(1 - Kappa).
+// Dry air gives Rm = Rd, Cpml = Cpd.
 __global__ void k_weights(long N, const double* rho, const double* rth, const double* T,
+                          const double* rhov, const double* rhol, const double* rhor,
                           double* theta, double* dut) {
   long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (c >= N) return;
   theta[c] = rth[c] / (rho[c] + kEps);
-  // WeightDiagCompute 'Density', dry air (RhoV = RhoL = 0): Rm = Rd, Cpml = Cpd.
-  const double kappa = kRd / kCpd;
-  const double rhoD = rho[c] - 0.0 - 0.0 + kEps;
-  dut[c] = T[c] / (rth[c] + kEps) * (rhoD * kRd + 0.0) / (1.0 - kappa);
+  const double rv = rhov ? rhov[c] : 0.0;
+  const double rl = (rhol ? rhol[c] : 0.0) + (rhor ? rhor[c] : 0.0);
+  const double rhoD = rho[c] - rv - rl + kEps;
+  const double Rm = kRd + kRv * rv / rhoD;
+  const double Cpml = kCpd + kCpv * rv / rhoD + kCpl * rl / rhoD;
+  const double kappa = Rm / Cpml;
+  dut[c] = T[c] / (rth[c] + kEps) * (rhoD * kRd + rv * kRv) / (1.0 - kappa);
 }
 
 void launch_weights(hipStream_t s, const Geom& g, const double* rho, const double* rth,
-                    const double* T, double* theta, double* dut) {
+                    const double* T, const double* rhov, const double* rhol,
+                    const double* rhor, double* theta, double* dut) {
   long N = g.N;
-  hipLaunchKernelGGL(k_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rho, rth, T, theta,
-                     dut);
+  hipLaunchKernelGGL(k_weights, dim3((N + 255) / 256), dim3(256), 0, s, N, rho, rth, T, rhov,
+                     rhol, rhor, theta, dut);
 }
 
 // Volume-weighted face theta (CellToFaceRhoWeight); wall faces copy the adjacent cell.

@@ -83,6 +83,16 @@ module asam_sound_iface
       real(c_double), intent(in) :: rho(*), rhotheta(*), T(*)
     end function
 
+    integer(c_int) function asam_sound_setup_moist(ctx, FU, FV, FW, VolC, dx, dy, dz, rho, &
+                                                   rhotheta, T, rhov, rhol, rhor) &
+        bind(c, name="asam_sound_setup_moist")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: FU(*), FV(*), FW(*), VolC(*), dx(*), dy(*), dz(*)
+      real(c_double), intent(in) :: rho(*), rhotheta(*), T(*)
+      type(c_ptr), value :: rhov, rhol, rhor
+    end function
+
     integer(c_int) function asam_sound_setup_weights(ctx, FU, FV, FW, VolC, dx, dy, dz, &
                                                      DTU, DTV, DTW, DUT) &
         bind(c, name="asam_sound_setup_weights")

@@ -70,14 +70,30 @@ contains
     if (missing) AsamStatus = ASAM_E_INVALID
   end subroutine AsamSoundInit
 
-  !> Operator set-up from geometry and thermodynamic state (JacAccGrav).
-  subroutine AsamJacAccGrav(FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T)
+  !> Operator set-up from geometry and thermodynamic state (JacAccGrav).  RhoV, RhoL and
+  !> RhoR (vapour, cloud water, rain densities of the interior cells) make the weights those
+  !> of moist air, as WeightDiagCompute 'Density' forms them; absent ones count as zero.
+  subroutine AsamJacAccGrav(FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T, RhoV, RhoL, RhoR)
     real(c_double), intent(in) :: FU(:,:,:), FV(:,:,:), FW(:,:,:), VolC(:,:,:)
     real(c_double), intent(in) :: dx(:), dy(:), dz(:)
     real(c_double), intent(in) :: Rho(:,:,:), RhoTheta(:,:,:), T(:,:,:)
+    real(c_double), intent(in), optional, target, contiguous :: RhoV(:,:,:), RhoL(:,:,:), &
+                                                                 RhoR(:,:,:)
     integer(c_int) :: rc
-    rc = asam_sound_setup(ctx, FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T)
-    call check(rc, 'asam_sound_setup')
+    type(c_ptr) :: pv, pl, pr
+    if (.not. (present(RhoV) .or. present(RhoL) .or. present(RhoR))) then
+      rc = asam_sound_setup(ctx, FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T)
+      call check(rc, 'asam_sound_setup')
+      return
+    end if
+    pv = c_null_ptr
+    pl = c_null_ptr
+    pr = c_null_ptr
+    if (present(RhoV)) pv = c_loc(RhoV)
+    if (present(RhoL)) pl = c_loc(RhoL)
+    if (present(RhoR)) pr = c_loc(RhoR)
+    rc = asam_sound_setup_moist(ctx, FU, FV, FW, VolC, dx, dy, dz, Rho, RhoTheta, T, pv, pl, pr)
+    call check(rc, 'asam_sound_setup_moist')
   end subroutine AsamJacAccGrav
 
   !> SolveSound(x,b,MaxIter,Tol): solves MatVec1(x) = -A x = b (Iter_Mod.f90:9-30).

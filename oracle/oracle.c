@@ -43,6 +43,9 @@
 #define EPS 1.0e-30
 #define RD (1004.0 - 717.0)
 #define CPD 1004.0
+#define RV 461.0   /* Parameter_Mod.f90:25-29 */
+#define CPV 1885.0
+#define CPL 4186.0
 
 struct orc_op {
   orc_grid g;
@@ -73,15 +76,27 @@ int orc_get_threads(void) {
 /* ------------------------------------------------------------------ weights */
 void orc_weights(const orc_grid *g, const double *V, const double *rho, const double *rth,
                  const double *T, double *DTU, double *DTV, double *DTW, double *DUT) {
+  orc_weights_moist(g, V, rho, rth, T, NULL, NULL, NULL, DTU, DTV, DTW, DUT);
+}
+
+void orc_weights_moist(const orc_grid *g, const double *V, const double *rho,
+                       const double *rth, const double *T, const double *rhov,
+                       const double *rhol, const double *rhor, double *DTU, double *DTV,
+                       double *DTW, double *DUT) {
   const int nx = g->nx, ny = g->ny, nz = g->nz;
   const long N = (long)nx * ny * nz;
   double *th = (double *)malloc(sizeof(double) * N);
   for (long c = 0; c < N; ++c) th[c] = rth[c] / (rho[c] + EPS);
-  /* DUT, JacAccGrav_Mod.f90:4262-4270 with RhoV=RhoL=0 (dry): Rm=Rd, Cpml=Cpd */
-  const double kappa = RD / CPD;
+  /* DUT, JacAccGrav_Mod.f90:4258-4268: RhoLLoc = RhoL + RhoR, RhoD = Rho - RhoV - RhoL + Eps,
+   * Rm = Rd + Rv RhoV/RhoD, Cpml = Cpd + Cpv RhoV/RhoD + Cpl RhoL/RhoD, Kappa = Rm/Cpml */
   for (long c = 0; c < N; ++c) {
-    double rhoD = rho[c] - 0.0 - 0.0 + EPS;
-    DUT[c] = T[c] / (rth[c] + EPS) * (rhoD * RD + 0.0) / (1.0 - kappa);
+    const double rv = rhov ? rhov[c] : 0.0;
+    const double rl = (rhol ? rhol[c] : 0.0) + (rhor ? rhor[c] : 0.0);
+    const double rhoD = rho[c] - rv - rl + EPS;
+    const double Rm = RD + RV * rv / rhoD;
+    const double Cpml = CPD + CPV * rv / rhoD + CPL * rl / rhoD;
+    const double kappa = Rm / Cpml;
+    DUT[c] = T[c] / (rth[c] + EPS) * (rhoD * RD + rv * RV) / (1.0 - kappa);
   }
   /* Face theta: volume-weighted mean of the two cells (JacAccGrav_Mod.f90:3316-3319);
    * wall faces take the adjacent cell's theta (they carry zero weight, SetFactorN). */

@@ -44,6 +44,13 @@ void orc_weights(const orc_grid *g, const double *VolC, const double *rho,
                  const double *rhotheta, const double *T, double *DTU, double *DTV,
                  double *DTW, double *DUT);
 
+/* Moist form (WeightDiagCompute 'Density', JacAccGrav_Mod.f90:4254-4270): vapour rhov,
+ * cloud water rhol and rain rhor densities (any may be NULL); rho is the total density. */
+void orc_weights_moist(const orc_grid *g, const double *VolC, const double *rho,
+                       const double *rhotheta, const double *T, const double *rhov,
+                       const double *rhol, const double *rhor, double *DTU, double *DTV,
+                       double *DTW, double *DUT);
+
 /* Export the nested tri-diagonal coefficients (x-fastest, face (c,c+e) stored at c).
  * T: d[N],ux,lx,uy,ly,uz,lz[N].  TB: d[4N] (2x2 column-major), uz,lz[4N], ux,lx,uy,ly[2N]. */
 void orc_export(const orc_op *A, double *d, double *ux, double *lx, double *uy, double *ly,

@@ -48,6 +48,7 @@ def lib():
         L.orc_assemble_weights.restype = C.c_void_p
         L.orc_assemble_weights.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 11
         L.orc_weights.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 8
+        L.orc_weights_moist.argtypes = [C.POINTER(OrcGrid)] + [_dp] * 11
         L.orc_free.argtypes = [C.c_void_p]
         L.orc_n.restype = C.c_long
         L.orc_n.argtypes = [C.c_void_p]
@@ -156,6 +157,21 @@ class Oracle:
         DUT = np.empty((g.nz, g.ny, g.nx))
         lib().orc_weights(C.byref(self.g), _p(_c(p.VolC)), _p(_c(p.rho)), _p(_c(p.rhotheta)),
                           _p(_c(p.T)), _p(DTU), _p(DTV), _p(DTW), _p(DUT))
+        return DTU, DTV, DTW, DUT
+
+    def weights_moist(self, rhov=None, rhol=None, rhor=None):
+        """Face theta and DUT of moist air (WeightDiagCompute 'Density') for this problem's
+        state with vapour, cloud water and rain densities (None: none)."""
+        p, g = self.prob, self.prob.grid
+        DTU = np.empty((g.nz, g.ny, g.nx + 1))
+        DTV = np.empty((g.nz, g.ny + 1, g.nx))
+        DTW = np.empty((g.nz + 1, g.ny, g.nx))
+        DUT = np.empty((g.nz, g.ny, g.nx))
+        q = [None if a is None else _c(a) for a in (rhov, rhol, rhor)]
+        lib().orc_weights_moist(C.byref(self.g), _p(_c(p.VolC)), _p(_c(p.rho)),
+                                _p(_c(p.rhotheta)), _p(_c(p.T)),
+                                *[_p(a) if a is not None else None for a in q],
+                                _p(DTU), _p(DTV), _p(DTW), _p(DUT))
         return DTU, DTV, DTW, DUT
 
     def export(self):

@@ -369,3 +369,59 @@ def test_autotuned_shapes_keep_parity(precision, oracle_mod, sound_mod):
     else:
         rr = np.linalg.norm(p.b - orc.matvec1(x)) / np.linalg.norm(p.b)
         assert rr <= 1.0001e-8
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_moist_setup_matches_oracle(np1, oracle_mod, sound_mod):
+    """asam_sound_setup_moist (vapour, cloud water, rain) gives the oracle's moist operator:
+    MatVec1 to round-off, the FP64 solve to 1e-10 with iterations +-1; without moisture it
+    is the dry set-up bit for bit."""
+    p = small(np1)
+    g = p.grid
+    shape = (g.nz, g.ny, g.nx)
+    rv = 0.015 * p.rho * W.uniform01(g.ncell, 81).reshape(shape)
+    rl = 0.003 * p.rho * W.uniform01(g.ncell, 82).reshape(shape)
+    rr = 0.002 * p.rho * W.uniform01(g.ncell, 83).reshape(shape)
+    orc = oracle_mod.Oracle(p, weights=oracle_mod.Oracle(p).weights_moist(rv, rl, rr))
+    s = sound_mod.SoundSolver(g.nx, g.ny, g.nz, np1, p.grad_full, p.beta0_dt, p.grav)
+    s.setup_moist(p.FU, p.FV, p.FW, p.VolC, p.dxv, p.dyv, p.dzv, p.rho, p.rhotheta, p.T,
+                  rv, rl, rr)
+    x = live(p, W.normal(p.n, 84))
+    e = rel(s.MatVec1(x), orc.matvec1(x))
+    xo, mo, _, _ = orc.bicgstab(p.b, 300, 1e-8)
+    xg, mg, _, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
+    es = rel(xg, xo)
+    print(f"np1={np1} moist: MatVec1 {e:.2e}, solve {es:.2e}, MaxIter {mg} (oracle {mo})")
+    assert e < OP_TOL and es < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
+    s.setup_moist(p.FU, p.FV, p.FW, p.VolC, p.dxv, p.dyv, p.dzv, p.rho, p.rhotheta, p.T)
+    dry = sound_mod.SoundSolver.from_problem(p)
+    assert np.array_equal(s.MatVec1(x), dry.MatVec1(x))
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
+    """Every apply variant (z-march, one cell per thread, band order with one band per XCD
+    or 8-row bands) gives the oracle's MatVec1 to round-off and the FP64 solve to 1e-10,
+    with the residual form of the refinement step and the all-zero-guess form included
+    (mixed solve to 1e-8)."""
+    p = small(np1, nx=70, ny=37, nz=11)
+    orc = oracle_mod.Oracle(p)
+    xo, mo, _, _ = orc.bicgstab(p.b, 300, 1e-8)
+    x = live(p, W.normal(p.n, 85))
+    s = sound_mod.SoundSolver.from_problem(p)
+    for var, bx, rows, wg in ((0, 256, 0, 4096), (1, 64, 0, 4096), (2, 256, 0, 512),
+                              (2, 64, 8, 1024), (2, 128, 16, 256)):
+        s.set_tuning(apply_variant=var, apply_bx=bx, apply_band_rows=rows, apply_wg=wg)
+        e = rel(s.MatVec1(x), orc.matvec1(x))
+        et = rel(s.MatVec1_T(x), orc.matvec1_T(x))
+        s.set_control(precision="FP64")
+        xg, mg, _, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
+        s.set_control(precision="MIXED")
+        xm, _, _, im = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
+        rr = np.linalg.norm(p.b - orc.matvec1(xm)) / np.linalg.norm(p.b)
+        print(f"np1={np1} variant {var} bx {bx} rows {rows} wg {wg}: MatVec1 {e:.2e} "
+              f"transposed {et:.2e}, FP64 solve {rel(xg, xo):.2e} MaxIter {mg} (oracle {mo}), "
+              f"mixed true residual {rr:.2e}")
+        assert e < OP_TOL and et < OP_TOL
+        assert rel(xg, xo) < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
+        assert im.status == 0 and rr <= 1.0001e-8

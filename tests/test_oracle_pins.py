@@ -352,3 +352,50 @@ def test_rhs_divergence_theorem(oracle_mod):
     s2 = b[1::2].sum() + (Vc * ir.reshape(-1)).sum()
     ref = p.beta0_dt * np.abs(U).max() * p.FU.max() * p.grid.ncell
     assert abs(s1) <= 1e-13 * ref * 300 and abs(s2) <= 1e-13 * ref
+
+
+def moist_state(p, seed=71):
+    """Vapour, cloud water and rain densities for p's state: up to 1.5 %, 0.3 % and 0.2 % of
+    the density."""
+    g = p.grid
+    shape = (g.nz, g.ny, g.nx)
+    rv = 0.015 * p.rho * W.uniform01(g.ncell, seed).reshape(shape)
+    rl = 0.003 * p.rho * W.uniform01(g.ncell, seed + 1).reshape(shape)
+    rr = 0.002 * p.rho * W.uniform01(g.ncell, seed + 2).reshape(shape)
+    return rv, rl, rr
+
+
+def moist_eos(rho, rv, rl, rr, T, p0=1.0e5):
+    """First-principles moist air: partial pressures (Dalton), kappa = sum(rho_i R_i) /
+    sum(rho_i cp_i) over dry air, vapour and condensate, p = p0 (R rhotheta / p0)^(1/(1-kappa))
+    with R the gas constant per unit total mass.  Returns (rhotheta, p(rhotheta))."""
+    Rd, Rv, Cpd, Cpv, Cpl = 287.0, 461.0, 1004.0, 1885.0, 4186.0
+    rd = rho - rv - rl - rr
+    kappa = (rd * Rd + rv * Rv) / (rd * Cpd + rv * Cpv + (rl + rr) * Cpl)
+    R = (rd * Rd + rv * Rv) / rho
+    pres = (rd * Rd + rv * Rv) * T
+    rth = p0 / R * (pres / p0) ** (1.0 - kappa)
+
+    def P(x):
+        return p0 * (R * x / p0) ** (1.0 / (1.0 - kappa))
+    return rth, P
+
+
+def test_moist_dut_is_eos_derivative(oracle_mod):
+    """DUT of the moist 'Density' weights equals d p / d(rho*theta) of moist air at fixed
+    composition (central differences), and with no moisture the dry weights."""
+    p = W.config(4, np1=1, scale=0.25)
+    rv, rl, rr = moist_state(p)
+    rth, P = moist_eos(p.rho, rv, rl, rr, p.T)
+    q = W.SoundProblem(**{**p.__dict__, "rhotheta": rth})
+    orc = oracle_mod.Oracle(q)
+    _, _, _, dut = orc.weights_moist(rv, rl, rr)
+    h = 1e-6 * rth
+    fd = (P(rth + h) - P(rth - h)) / (2 * h)
+    err = np.max(np.abs(dut - fd) / fd)
+    print(f"moist DUT vs central differences of the equation of state: {err:.2e}")
+    assert err < 1e-8
+    # dry weights = moist weights without moisture; moisture changes DUT
+    d0 = orc.weights()[3]
+    assert np.array_equal(d0, orc.weights_moist()[3])
+    assert np.max(np.abs(dut - d0) / d0) > 1e-3
