@@ -227,3 +227,23 @@ def test_krylov_with_pressurepre(np1):
     assert res < 2e-10 and rel(orc.matvec1(xb), p.b) < 2e-10
     assert rel(xm, xl) < 1e-8
     assert itm < itl
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+@pytest.mark.parametrize("ref", [1, 4])
+def test_config1_iteration_counts(np1, ref):
+    """The 64×64×40 cut-cell stage system (10 km × 10 km × 10 km, Agnesi ridge): FGMRES(10)
+    to 1e-8 takes 9 (T) or 10 (TB) steps with the vertical-line preconditioner and 2 with
+    the reference's PressurePre (RefLevel 1 and 4); both reach the same solution."""
+    p = W.config(1, np1=np1)
+    orc = P.Oracle(p)
+    xl, itl, _ = orc.fgmres(p.b, 20, 1e-8, 10)
+    M = P.PressurePre(orc, ref_level=ref)
+    M.attach()
+    xm, itm, _ = orc.fgmres(p.b, 20, 1e-8, 10)
+    M.detach()
+    res = rel(orc.matvec1(xm), p.b)
+    print(f"np1={np1} RefLevel {ref}: FGMRES steps line {itl}, PressurePre {itm}; "
+          f"true rel residual {res:.1e}; |x_mg - x_line|/|x_line| {rel(xm, xl):.1e}")
+    assert res < 1e-8 and rel(xm, xl) < 1e-7
+    assert itl in (9, 10) and itm == 2
