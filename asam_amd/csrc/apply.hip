@@ -242,6 +242,7 @@ __global__ __launch_bounds__(256) void k_apply(KApply<CT, VX, VO, VU> a) {
   }
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
+    if (a.tail.kind >= 0) tail_arrive(a.tail, tid);
   }
 }
 
@@ -271,6 +272,7 @@ __global__ __launch_bounds__(256) void k_apply_band(KApply<CT, VX, VO, VU> a) {
   }
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
+    if (a.tail.kind >= 0) tail_arrive(a.tail, tid);
   }
 }
 
@@ -285,6 +287,7 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
   march_tile<NP, TR, FL, CT, VX, VO, VU>(a, L, acc);
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
+    if (a.tail.kind >= 0) tail_arrive(a.tail, tid);
   }
 }
 
@@ -298,6 +301,7 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
   a.x = (const VX*)r.x; a.dx = r.dx; a.xw = r.xw; a.b = r.b; a.y = (VO*)r.y;
   a.y2 = (VO*)r.y2; a.u = (const VU*)r.u; a.partial = r.partial; a.done = r.done;
   a.dead = r.x0_dead;
+  a.tail = r.tail;
   a.band_rows = band_rows(g);
   if (g.tune.apply_variant == 2) {
     // the x = 0 form too, so that it writes the partial sums of the full form

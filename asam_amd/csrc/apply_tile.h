@@ -41,6 +41,7 @@ struct KApply {
   double* partial;
   const int* done;
   const unsigned char* dead;  // AF_X0: cells whose residual is 0
+  KTail tail;                 // fused scalar tail of the dots (kind < 0: none)
 };
 
 // Loads of arrays each cell reads exactly once (no neighbour reuse): nontemporal, so they

@@ -28,7 +28,9 @@ __device__ __forceinline__ void block_reduce_store_b(double (&v)[NQ], double* pa
     for (int q = 0; q < NQ; ++q) {
       double s = 0.0;
       for (int w = 0; w < nw; ++w) s += sh[q][w];
-      partial[q * kRedBlocks + blk] = s;
+      // written through to the device-coherent level, for a fused tail (kcommon.h)
+      __hip_atomic_store(partial + q * kRedBlocks + blk, s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -47,7 +49,7 @@ __global__ __launch_bounds__(256) void k_bicg_update(long n, VT* __restrict__ x,
                                                      const VT* __restrict__ rh,
                                                      VT* __restrict__ r, const VT* __restrict__ t,
                                                      const VT* __restrict__ rt, const KState* st,
-                                                     double* partial) {
+                                                     double* partial, KTail tail) {
   if (st->done) return;
   const double alpha = st->alpha, omega = st->omega;
   const bool half = st->half != 0;
@@ -61,20 +63,21 @@ __global__ __launch_bounds__(256) void k_bicg_update(long n, VT* __restrict__ x,
       bicg_upd_elem<VT>(e, x, ph, rh, r, t, rt, alpha, omega, acc);
   }
   block_reduce_store_b<2>(acc, partial, blockIdx.x);
+  if (tail.kind >= 0) tail_arrive(tail, threadIdx.x);
 }
 
 void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,
                         void* r, const void* t, const void* rt, const KState* st,
-                        double* partial) {
+                        double* partial, const KTail& tail) {
   int nb = blas_blocks(n);
   if (vt == 0)
     hipLaunchKernelGGL(k_bicg_update<double>, dim3(nb), dim3(256), 0, s, n, (double*)x,
                        (const double*)ph, (const double*)rh, (double*)r, (const double*)t,
-                       (const double*)rt, st, partial);
+                       (const double*)rt, st, partial, tail);
   else
     hipLaunchKernelGGL(k_bicg_update<float>, dim3(nb), dim3(256), 0, s, n, (float*)x,
                        (const float*)ph, (const float*)rh, (float*)r, (const float*)t,
-                       (const float*)rt, st, partial);
+                       (const float*)rt, st, partial, tail);
 }
 
 // ------------------------------------------------------------- (F)GMRES MGS

@@ -45,6 +45,8 @@ struct Tuning {
   int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
   int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
+  int fused_tail = 0;     // 1: BiCGStab and refinement tails run by the producing kernel's
+                          // last workgroup (0: a separate finalize launch each)
 };
 
 struct Geom {
@@ -93,6 +95,18 @@ struct KState {
   double cs[kMaxM], sn[kMaxM], bhat[kMaxM + 1], y[kMaxM], vscale[kMaxM + 1];
 };
 
+// Scalar tail of a reduction run by the last workgroup of the producing kernel in place
+// of a finalize launch (kind < 0: none).  The kinds are those whose step reads and
+// writes only the state's scalars ahead of H.  counter counts the arriving workgroups;
+// it is zero between launches (the last workgroup resets it).
+struct KTail {
+  int kind = -1;
+  int nq = 0, nblk = 0, nblk_hi = 0;  // slots, partials of slots 0-1 and of slots 2-3
+  const double* partial = nullptr;
+  KState* st = nullptr;
+  unsigned int* counter = nullptr;
+};
+
 // ------------------------------------------------------------------ set-up
 void launch_weights(hipStream_t s, const Geom& g, const double* rho, const double* rth,
                     const double* T, const double* rhov, const double* rhol,
@@ -128,6 +142,7 @@ struct ApplyReq {
   // x known to be zero (x unused): the residual form is y = b, zero on the cells this
   // mask marks dead (their operator row is empty), the same values as the full apply
   const unsigned char* x0_dead = nullptr;
+  KTail tail;                  // fused scalar tail of the dots (needs dots)
 };
 int apply_blocks(const Geom& g);
 // dead[c] = 1 where the residual form writes 0 (no volume and no face coefficient), from
@@ -169,7 +184,7 @@ void launch_any_nonzero(hipStream_t s, long n, const double* x, int* xnz, int* x
 // With st->half set: x += alpha pHat only.
 void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,
                         void* r, const void* t, const void* rt, const KState* st,
-                        double* partial);
+                        double* partial, const KTail& tail = KTail());
 // (F)GMRES modified Gram-Schmidt step: w -= H[k-1][i]*vscale[k-1]*v_{k-1} (k>0); then
 // q0 = w.v_k (k<=i) or w.w (k==i+1), into partial slot 0 of `partial`.  hsrc: null (H[k-1][i]
 // is in the state), or the previous step's partial products, whose sum is H[k-1][i]

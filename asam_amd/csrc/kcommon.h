@@ -53,9 +53,100 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NQ], double* part
     for (int q = 0; q < NQ; ++q) {
       double s = 0.0;
       for (int w = 0; w < nw; ++w) s += sh[q][w];
-      partial[q * stride + blk] = s;
+      // written through to the device-coherent level (a fused tail in another XCD may
+      // read it before this kernel ends)
+      __hip_atomic_store(partial + q * stride + blk, s, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// Scalar words of the state ahead of the (F)GMRES arrays: all that the fused tails touch.
+constexpr int kTailWords = (int)(offsetof(KState, H) / 8);
+
+// Fused scalar tail, in place of a k_finalize launch.  Every thread of every workgroup
+// of 256 threads calls it right after block_reduce_store.
+//
+// Publication: thread 0 has written the workgroup's partial sums with agent-scope
+// (write-through) stores.  It waits for them to complete, then counts the workgroup in
+// with an agent-scope atomic add.  This is the order an agent-scope release gives atomic
+// stores; the release fence itself would also write back every dirty line of this XCD's
+// L2 in every workgroup, which costs far more than the finalize launch it replaces.
+//
+// The workgroup that arrives last invalidates its caches (agent-scope acquire, after its
+// atomic returned) and adds all partial sums in k_finalize's order: k_finalize's thread v
+// (of 1024) sums the partials v, v + 1024, ... in turn; the butterfly then adds each group
+// of 64 such sums; the 16 group totals are added in order.  Here thread t forms the sums
+// of v = t, t + 256, t + 512, t + 768 side by side (independent loads in flight), parks
+// them in LDS, and each wave runs the butterflies of groups w, w + 4, w + 8, w + 12, so
+// the scalars have k_finalize's bits.  It runs the scalar step on an LDS copy of the
+// state's scalars and resets the counter.  No workgroup waits for another.
+__device__ __forceinline__ void tail_arrive(const KTail& t, int tid) {
+  __shared__ int s_last;
+  __shared__ unsigned long long sk[kTailWords];
+  __shared__ double sh[4][16];
+  __shared__ double vs[1024];
+  if (tid == 0) {
+    // completion of this wave's stores, the partial sums included; also a compiler
+    // barrier, so the atomic add below is not issued before them
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const unsigned int total = gridDim.x * gridDim.y * gridDim.z;
+    const int last = __hip_atomic_fetch_add(t.counter, 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) == total - 1u ? 1 : 0;
+    // the barrier below orders the loads of the workgroup's other threads after this
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    s_last = last;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  for (int w = tid; w < kTailWords; w += 256)
+    sk[w] = reinterpret_cast<const unsigned long long*>(t.st)[w];
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int q = 0; q < 4; ++q) {
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (q < t.nq) {
+      const int nb = q < 2 ? t.nblk : t.nblk_hi;
+      const double* pq = t.partial + (long)q * kRedBlocks;
+      // full rounds of 1024 first (no lane condition, so their loads are in flight
+      // together), then the partial last round
+      const int full = nb >> 10;
+#pragma unroll 4
+      for (int r = 0; r < full; ++r) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s4[c] += pq[(r << 10) + tid + (c << 8)];
+      }
+      if ((full << 10) < nb) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int b = (full << 10) + tid + (c << 8);
+          if (b < nb) s4[c] += pq[b];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vs[tid + (c << 8)] = s4[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int g = wid + (c << 2);
+      const double v = wave_sum(vs[(g << 6) + lane]);
+      if (lane == 0) sh[q][g] = v;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double S[4];
+    for (int q = 0; q < 4; ++q) {
+      double s = 0.0;
+      for (int w = 0; w < 16; ++w) s += sh[q][w];
+      S[q] = q < t.nq ? s : 0.0;
+    }
+    fin_logic(t.kind, S, reinterpret_cast<KState*>(sk), 0, 0, t.nq, nullptr);
+    __hip_atomic_store(t.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  for (int w = tid; w < kTailWords; w += 256)
+    reinterpret_cast<unsigned long long*>(t.st)[w] = sk[w];
 }
 
 template <typename T> struct V2 { T a, b; };

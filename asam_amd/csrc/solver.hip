@@ -33,6 +33,7 @@ struct asam_sound_ctx {
   double *dxv = nullptr, *dyv = nullptr, *dzv = nullptr;    // cell spacings
   double* pj_scratch = nullptr;  // host-array rhs/update staging (allocated on first use)
   double* partial = nullptr;
+  unsigned int* tail_ctr = nullptr;  // arrival counter of the fused tails (zero at rest)
   unsigned char* dead = nullptr;  // cells whose residual is 0 (from the FP64 coefficients)
   KState* st = nullptr;
   KState* st_host = nullptr;  // pinned
@@ -126,7 +127,7 @@ extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning*
   t->apply_wg = c->g.tune.apply_wg;
   t->line_window = c->g.tune.line_window;
   t->line_lds = c->g.tune.line_lds;
-  t->fused_tail = 0;
+  t->fused_tail = c->g.tune.fused_tail;
   t->apply_xcd = c->g.tune.apply_xcd;
   t->apply_bx = c->g.tune.apply_bx;
   t->line_tiled = c->g.tune.line_tiled;
@@ -147,7 +148,8 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.line_window = t->line_window;
   if (t->line_lds < 0 || t->line_lds > 160) return fail(c, ASAM_E_INVALID, "line_lds");
   nt.line_lds = t->line_lds;
-  if (t->fused_tail) return fail(c, ASAM_E_INVALID, "fused_tail is not supported");
+  if (t->fused_tail != 0 && t->fused_tail != 1) return fail(c, ASAM_E_INVALID, "fused_tail");
+  nt.fused_tail = t->fused_tail;
   if (t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
     return fail(c, ASAM_E_INVALID, "apply_bx");
   nt.apply_xcd = t->apply_xcd ? 1 : 0;
@@ -230,6 +232,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
             (long)grid->nx * grid->ny * (grid->nz + 1);
   rc |= dalloc(c, &c->thfaces, nf);
   rc |= dalloc(c, &c->partial, 4L * kRedBlocks);
+  rc |= dalloc(c, &c->tail_ctr, 64);
   rc |= dalloc(c, &c->st, 1);
   rc |= dalloc(c, &c->dead, N);
   const long n = c->n;
@@ -240,6 +243,8 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
   if (rc) return fin(ASAM_E_NOMEM);
   HIPCHK(c, hipHostMalloc((void**)&c->st_host, sizeof(KState), hipHostMallocDefault));
   memset(c->st_host, 0, sizeof(KState));
+  HIPCHK(c, hipMemsetAsync(c->tail_ctr, 0, 64 * sizeof(unsigned int), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return fin(0);
 }
 
@@ -410,6 +415,20 @@ struct Enq {
   int iter = 0;  // BiCGStab iteration index of the launches being enqueued (profiling)
   explicit Enq(asam_sound_ctx* cc, int t) : c(cc), s(cc->stream), tr(t) {}
 
+  // The tails whose step touches only the state's scalars run inside the producing
+  // kernel when the tuning asks for it; the others keep their finalize launch.
+  bool fused(int kind) const {
+    return c->g.tune.fused_tail &&
+           (kind == FIN_BICG_INIT || kind == FIN_BICG_ALPHA || kind == FIN_BICG_HALF_OMEGA ||
+            kind == FIN_BICG_END || kind == FIN_MIXED_OUTER);
+  }
+  KTail tail(int kind, int nq, int nblk, int nblk_hi) const {
+    KTail t;
+    t.kind = kind; t.nq = nq; t.nblk = nblk; t.nblk_hi = nblk_hi > 0 ? nblk_hi : nblk;
+    t.partial = c->partial; t.st = c->st; t.counter = c->tail_ctr;
+    return t;
+  }
+
   hipEvent_t ev() {
     if (c->ev_next >= c->ev_pool.size()) {
       hipEvent_t e = nullptr;
@@ -431,19 +450,23 @@ struct Enq {
     c->prof_pending.push_back({cls, tag, iter, bytes, half_bytes, e0, e1});
   }
 
-  // Launch a kernel, then (kind >= 0) the finalize of its reduction.
+  // Launch a kernel and (kind >= 0) the tail of its reduction: fused into the kernel, or
+  // a finalize launch after it.  fnblk_hi: workgroups behind partial slots 2-3.
   void apply(int prec, int vx, int vo, int vu, const void* x, void* y, const double* b,
              void* y2, const void* u, int dots, const int* done, const float* dx = nullptr,
-             double* xw = nullptr, int tag = 0, int fkind = -1, int fnq = 0) {
+             double* xw = nullptr, int tag = 0, int fkind = -1, int fnq = 0,
+             int fnblk_hi = 0) {
     ApplyReq r;
     r.tr = tr; r.prec = prec; r.vx = vx; r.vo = vo; r.vu = vu; r.x = x; r.y = y; r.b = b;
     r.y2 = y2; r.u = u; r.dots = dots; r.partial = c->partial; r.done = done; r.dx = dx;
     r.xw = xw;
+    const bool fuse = fkind >= 0 && dots && fused(fkind);
+    if (fuse) r.tail = tail(fkind, fnq, apply_blocks(c->g), fnblk_hi);
     timed(prec ? 1 : 0, apply_bytes(c->g, r),
           [&] { c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0; },
           tag);
     c->counts[0]++;
-    if (fkind >= 0) fin(fkind, fnq, apply_blocks(c->g), 0, 0, tag);
+    if (fkind >= 0 && !fuse) fin(fkind, fnq, apply_blocks(c->g), 0, 0, tag, fnblk_hi);
   }
   // Initial residual y = y2 = b - Op(x), y.y into the partial sums, then finalize fkind.
   // Unprofiled, a scan of x picks on the device which of two launches writes y: the full
@@ -461,11 +484,15 @@ struct Enq {
     ApplyReq r;
     r.tr = tr; r.prec = 0; r.vx = 0; r.vo = vo; r.vu = 0; r.x = x; r.y = y; r.b = b;
     r.y2 = y2; r.dots = dots; r.partial = c->partial; r.done = &c->st->xz;
+    // exactly one of the two launches runs (the other returns on its flag), so at most
+    // one of them arrives at the fused tail
+    const bool fuse = fkind >= 0 && dots && fused(fkind);
+    if (fuse) r.tail = tail(fkind, 1, apply_blocks(c->g), 0);
     c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     r.x = nullptr; r.x0_dead = c->dead; r.done = &c->st->xnz;
     c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     c->counts[0]++;
-    if (fkind >= 0) fin(fkind, 1, apply_blocks(c->g));
+    if (fkind >= 0 && !fuse) fin(fkind, 1, apply_blocks(c->g));
   }
   void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
             const double* scale, const int* done, int qslot = 0) {
@@ -487,11 +514,13 @@ struct Enq {
   void bicg_update(int vt, void* x, const void* ph, const void* rh, void* r, const void* t,
                    const void* rt) {
     const double w = vt ? 4 : 8, nv = (double)c->n;
+    const bool fuse = fused(FIN_BICG_END);
+    const KTail tl = fuse ? tail(FIN_BICG_END, 2, blas_blocks(c->n), 0) : KTail();
     timed(4 + vt, 8 * nv * w,
-          [&] { launch_bicg_update(s, c->n, vt, x, ph, rh, r, t, rt, c->st, c->partial); }, 2,
-          3 * nv * w);
+          [&] { launch_bicg_update(s, c->n, vt, x, ph, rh, r, t, rt, c->st, c->partial, tl); },
+          2, 3 * nv * w);
     c->counts[2]++;
-    fin(FIN_BICG_END, 2, blas_blocks(c->n));
+    if (!fuse) fin(FIN_BICG_END, 2, blas_blocks(c->n));
   }
   // classical Gram-Schmidt: nq projections of w on basis vectors k0.., then the removal
   // of all i + 1 projections with |w|^2
@@ -568,7 +597,9 @@ int upload_state(asam_sound_ctx* c) {
   return 0;
 }
 
-// One BiCGStab iteration (SpMatrix4CellPar_Mod.f90:280-337) as 5 fused kernels + 4 tails.
+// One BiCGStab iteration (SpMatrix4CellPar_Mod.f90:280-337) as 5 fused kernels; the
+// scalar tails (alpha; half-step test and omega; exit tests and beta) run in the last
+// workgroup of the kernel before them, or as finalize launches (fused_tail = 0).
 // prec/vt: 0 FP64, 1 FP32 vectors and coefficients (dots always FP64).
 template <typename VT>
 void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v, VT* rh, VT* t) {
@@ -584,8 +615,8 @@ void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v
   // s = r - alpha v (into r); |s| (partial slot 2); rHat = PSolve(s)
   e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done, 2);
   // t = MatVec(rHat); t.t, t.s; then the half-step exit test on |s| or omega = (t.s)/(t.t)
-  e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->done);
-  e.fin(FIN_BICG_HALF_OMEGA, 3, apply_blocks(g), 0, 0, 0, line_blocks(g));
+  e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->done, nullptr,
+          nullptr, 0, FIN_BICG_HALF_OMEGA, 3, line_blocks(g));
   // x += alpha pHat + omega rHat; r = s - omega t (half exit: x += alpha pHat);
   // |r|, rTilde.r; exit tests, beta
   e.bicg_update(vt, x, ph, rh, r, t, rt);
@@ -1409,6 +1440,16 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
       if (rc) { c->g.tune = t0; return rc; }
       if (m < best_ms) { best_ms = m; bt = c->g.tune; }
     }
+  // scalar tails in the producing kernels' last workgroups, or as finalize launches
+  const Tuning bl = bt;
+  for (int ft : {0, 1}) {
+    c->g.tune = bl;
+    c->g.tune.fused_tail = ft;
+    double m = 0;
+    int rc = measure(&m);
+    if (rc) { c->g.tune = t0; return rc; }
+    if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+  }
   c->g.tune = bt;
   memset(c->st_host, 0, sizeof(KState));
   if (best) asam_sound_get_tuning(c, best);

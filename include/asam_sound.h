@@ -101,7 +101,11 @@ typedef struct {
                         (staged in the output vector), 1 default (60 KiB), 2..160 that
                         many KiB; the lowest levels that fit are staged in LDS, the rest
                         in the output vector */
-  int fused_tail;    /* reserved, must be 0 (reductions end in a separate finalize launch) */
+  int fused_tail;    /* 1: the scalar steps of BiCGStab and of the mixed refinement run
+                        in the last workgroup of the kernel that produced their sums;
+                        0 (default): a separate finalize launch each.  Same bits either
+                        way; the autotuner times both.  (F)GMRES tails are always
+                        separate launches */
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
   int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
   int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
@@ -116,7 +120,8 @@ int asam_sound_set_control(asam_sound_ctx *ctx, const asam_sound_control *ctl);
 int asam_sound_get_tuning(const asam_sound_ctx *ctx, asam_sound_tuning *t);
 int asam_sound_set_tuning(asam_sound_ctx *ctx, const asam_sound_tuning *t);
 /* Launch-shape autotuner: for every candidate shape of the stencil apply (variant, width,
- * XCD order, workgroup count), then of the line solve (window, LDS staging), times whole
+ * XCD order, workgroup count), then of the line solve (window, LDS staging), then with
+ * fused and with separate scalar tails (fused_tail), times whole
  * BiCGStab iterations (all kernels and finalizes in solve order) on this context's operator
  * in the precision the control selects (FP32 kernels for ASAM_PREC_MIXED), 2*reps
  * iterations each, and keeps the fastest.  best and ms[2] (ms per iteration of the chosen
