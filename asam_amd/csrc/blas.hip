@@ -6,6 +6,7 @@
 // one-workgroup finalize kernel that sums the partials in a fixed order, so a
 // dot product is bitwise reproducible run to run.
 #include <math.h>
+#include <stdint.h>
 
 #include "kcommon.h"
 
@@ -97,9 +98,64 @@ __device__ double block_sum_partials(const double* p, int nb) {
   return total;
 }
 
+// 16-byte packs of a vector: W = 4 floats or 2 doubles per load and store.  The FP32
+// passes of the Gram-Schmidt chains are latency-bound with one element per lane (half the
+// bytes in flight of the FP64 passes at the same speed), so the kernels below stream
+// packs when every vector they touch is 16-byte aligned.
+template <typename VT>
+struct Pk {
+  static constexpr int W = 16 / (int)sizeof(VT);
+  VT v[W];
+};
+template <typename VT>
+__device__ __forceinline__ Pk<VT> ldpk(const VT* p, long c) {
+  Pk<VT> r;
+  if constexpr (sizeof(VT) == 4) {
+    const float4 t = reinterpret_cast<const float4*>(p)[c];
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+  } else {
+    const double2 t = reinterpret_cast<const double2*>(p)[c];
+    r.v[0] = t.x; r.v[1] = t.y;
+  }
+  return r;
+}
+template <typename VT>
+__device__ __forceinline__ void stpk(VT* p, long c, const Pk<VT>& r) {
+  if constexpr (sizeof(VT) == 4)
+    reinterpret_cast<float4*>(p)[c] = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  else
+    reinterpret_cast<double2*>(p)[c] = make_double2(r.v[0], r.v[1]);
+}
+template <typename VT>
+__device__ __forceinline__ Pk<VT> zpk() {
+  Pk<VT> r;
+#pragma unroll
+  for (int j = 0; j < Pk<VT>::W; ++j) r.v[j] = (VT)0;
+  return r;
+}
+__host__ inline bool al16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+// One element of a modified Gram-Schmidt pass: w -= h (v_prev * sp) (k > 0), rounded to
+// the vector type; then the product with v_k * sk, or w.w on the step's last pass.
+template <typename VT>
+__device__ __forceinline__ VT mgs_elem(VT w, VT vp, VT vk, int k, bool self, double h,
+                                       double sp, double sk, double& acc) {
+  double wv = (double)w;
+  if (k > 0) {
+    wv = wv + (-h) * ((double)vp * sp);
+    w = (VT)wv;
+    wv = (double)w;
+  }
+  if (self) acc += wv * wv;
+  else acc += ((double)vk * sk) * wv;
+  return w;
+}
+
 // hsrc: null, or the partial products of the previous pass of the chain; then H(k-1, i) is
 // their sum, formed here (workgroup 0 records it) instead of by a finalize launch.
-template <typename VT>
+// PK: the vectors stream in 16-byte packs (every pointer aligned), the n mod W last
+// elements one per thread.
+template <typename VT, bool PK>
 __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
                                              const VT* __restrict__ vprev,
                                              const VT* __restrict__ vk, int k, int i,
@@ -120,31 +176,65 @@ __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
   if (!self) sk = st->vscale[k];
   double acc[1] = {0.0};
   const long stride = (long)gridDim.x * blockDim.x;
-  // U turns of the grid-stride loop at a time: the loads of all U elements first, then
-  // their arithmetic in the plain loop's order (same elements, same sums, same bits)
-  constexpr int U = 4;
-  for (long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x; e0 < n; e0 += U * stride) {
-    VT wl[U], pl[U], kl[U];
+  const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if constexpr (PK) {
+    constexpr int W = Pk<VT>::W, U = 2;
+    const long nc = n / W;
+    long c0 = gid;
+    // whole turns: U packs per thread, all loads first
+    for (; c0 + (U - 1) * stride < nc; c0 += U * stride) {
+      Pk<VT> wl[U], pl[U], kl[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long e = e0 + u * stride;
-      const bool ok = e < n;
-      wl[u] = ok ? w[e] : (VT)0;
-      pl[u] = (ok && k > 0) ? vprev[e] : (VT)0;
-      kl[u] = (ok && !self) ? vk[e] : (VT)0;
+      for (int u = 0; u < U; ++u) {
+        const long c = c0 + u * stride;
+        wl[u] = ldpk(w, c);
+        pl[u] = k > 0 ? ldpk(vprev, c) : zpk<VT>();
+        kl[u] = !self ? ldpk(vk, c) : zpk<VT>();
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          wl[u].v[j] = mgs_elem(wl[u].v[j], pl[u].v[j], kl[u].v[j], k, self, h, sp, sk, acc[0]);
+        if (k > 0) stpk(w, c0 + u * stride, wl[u]);
+      }
     }
+    for (; c0 < nc; c0 += stride) {
+      Pk<VT> wl = ldpk(w, c0);
+      const Pk<VT> pl = k > 0 ? ldpk(vprev, c0) : zpk<VT>();
+      const Pk<VT> kl = !self ? ldpk(vk, c0) : zpk<VT>();
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long e = e0 + u * stride;
-      if (e < n) {
-        double wv = (double)wl[u];
-        if (k > 0) {
-          wv = wv + (-h) * ((double)pl[u] * sp);
-          w[e] = (VT)wv;
-          wv = (double)(VT)wv;
+      for (int j = 0; j < W; ++j)
+        wl.v[j] = mgs_elem(wl.v[j], pl.v[j], kl.v[j], k, self, h, sp, sk, acc[0]);
+      if (k > 0) stpk(w, c0, wl);
+    }
+    const long e = nc * W + gid;
+    if (e < n) {
+      const VT r = mgs_elem(w[e], k > 0 ? vprev[e] : (VT)0, !self ? vk[e] : (VT)0, k, self, h,
+                            sp, sk, acc[0]);
+      if (k > 0) w[e] = r;
+    }
+  } else {
+    // U turns of the grid-stride loop at a time: the loads of all U elements first, then
+    // their arithmetic in the plain loop's order (same elements, same sums, same bits)
+    constexpr int U = 4;
+    for (long e0 = gid; e0 < n; e0 += U * stride) {
+      VT wl[U], pl[U], kl[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long e = e0 + u * stride;
+        const bool ok = e < n;
+        wl[u] = ok ? w[e] : (VT)0;
+        pl[u] = (ok && k > 0) ? vprev[e] : (VT)0;
+        kl[u] = (ok && !self) ? vk[e] : (VT)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long e = e0 + u * stride;
+        if (e < n) {
+          const VT r = mgs_elem(wl[u], pl[u], kl[u], k, self, h, sp, sk, acc[0]);
+          if (k > 0) w[e] = r;
         }
-        if (self) acc[0] += wv * wv;
-        else acc[0] += ((double)kl[u] * sk) * wv;
       }
     }
   }
@@ -154,48 +244,90 @@ __global__ __launch_bounds__(256) void k_mgs(long n, VT* __restrict__ w,
 void launch_mgs(hipStream_t s, long n, int vt, void* w, const void* vprev, const void* vk, int k,
                 int i, KState* st, double* partial, int chk_done, const double* hsrc) {
   const int nb = blas_blocks(n);
-  if (vt == 0)
-    hipLaunchKernelGGL(k_mgs<double>, dim3(nb), dim3(256), 0, s, n, (double*)w,
-                       (const double*)vprev, (const double*)vk, k, i, st, partial, chk_done,
-                       hsrc, nb);
-  else
-    hipLaunchKernelGGL(k_mgs<float>, dim3(nb), dim3(256), 0, s, n, (float*)w,
-                       (const float*)vprev, (const float*)vk, k, i, st, partial, chk_done,
-                       hsrc, nb);
+  const bool pk = al16(w) && (k == 0 || al16(vprev)) && (!vk || al16(vk));
+  if (vt == 0) {
+    auto f = pk ? k_mgs<double, true> : k_mgs<double, false>;
+    hipLaunchKernelGGL(f, dim3(nb), dim3(256), 0, s, n, (double*)w, (const double*)vprev,
+                       (const double*)vk, k, i, st, partial, chk_done, hsrc, nb);
+  } else {
+    auto f = pk ? k_mgs<float, true> : k_mgs<float, false>;
+    hipLaunchKernelGGL(f, dim3(nb), dim3(256), 0, s, n, (float*)w, (const float*)vprev,
+                       (const float*)vk, k, i, st, partial, chk_done, hsrc, nb);
+  }
 }
 
 struct ZList {
   const void* z[kMaxM];
 };
 
+// x += sum_q y_q z_q (times vscale_q for GMRES's unscaled basis), one element.
 template <typename VT>
+__device__ __forceinline__ double gm_upd_elem(double xv, double zq, int use_vscale,
+                                              double vsq, double yq) {
+  if (use_vscale) zq = zq * vsq;
+  return xv + yq * zq;
+}
+
+template <typename VT, bool PK>
 __global__ __launch_bounds__(256) void k_gmres_update(long n, VT* __restrict__ x, ZList zl,
                                                       int m, int use_vscale, const KState* st) {
   if (st->pad) return;  // pad doubles as "no update this cycle"
   const int nR = st->nR;
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    double xv = (double)x[e];
-    for (int q = 0; q < nR && q < m; ++q) {
-      double zq = (double)((const VT*)zl.z[q])[e];
-      if (use_vscale) zq = zq * st->vscale[q];
-      xv = xv + st->y[q] * zq;
+  const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if constexpr (PK) {
+    constexpr int W = Pk<VT>::W;
+    const long nc = n / W;
+    for (long c = gid; c < nc; c += stride) {
+      Pk<VT> xl = ldpk(x, c);
+      double xv[W];
+#pragma unroll
+      for (int j = 0; j < W; ++j) xv[j] = (double)xl.v[j];
+      for (int q = 0; q < nR && q < m; ++q) {
+        const Pk<VT> zp = ldpk((const VT*)zl.z[q], c);
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          xv[j] = gm_upd_elem<VT>(xv[j], (double)zp.v[j], use_vscale, st->vscale[q], st->y[q]);
+      }
+#pragma unroll
+      for (int j = 0; j < W; ++j) xl.v[j] = (VT)xv[j];
+      stpk(x, c, xl);
     }
-    x[e] = (VT)xv;
+    const long e = nc * W + gid;
+    if (e < n) {
+      double xv = (double)x[e];
+      for (int q = 0; q < nR && q < m; ++q)
+        xv = gm_upd_elem<VT>(xv, (double)((const VT*)zl.z[q])[e], use_vscale, st->vscale[q],
+                             st->y[q]);
+      x[e] = (VT)xv;
+    }
+  } else {
+    for (long e = gid; e < n; e += stride) {
+      double xv = (double)x[e];
+      for (int q = 0; q < nR && q < m; ++q)
+        xv = gm_upd_elem<VT>(xv, (double)((const VT*)zl.z[q])[e], use_vscale, st->vscale[q],
+                             st->y[q]);
+      x[e] = (VT)xv;
+    }
   }
 }
 
 void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs, int m,
                          int use_vscale, const KState* st) {
   ZList zl;
-  for (int q = 0; q < kMaxM; ++q) zl.z[q] = q < m ? zs[q] : nullptr;
+  bool pk = al16(x);
+  for (int q = 0; q < kMaxM; ++q) {
+    zl.z[q] = q < m ? zs[q] : nullptr;
+    if (q < m) pk = pk && al16(zs[q]);
+  }
   int nb = blas_blocks(n);
-  if (vt == 0)
-    hipLaunchKernelGGL(k_gmres_update<double>, dim3(nb), dim3(256), 0, s, n, (double*)x, zl, m,
-                       use_vscale, st);
-  else
-    hipLaunchKernelGGL(k_gmres_update<float>, dim3(nb), dim3(256), 0, s, n, (float*)x, zl, m,
-                       use_vscale, st);
+  if (vt == 0) {
+    auto f = pk ? k_gmres_update<double, true> : k_gmres_update<double, false>;
+    hipLaunchKernelGGL(f, dim3(nb), dim3(256), 0, s, n, (double*)x, zl, m, use_vscale, st);
+  } else {
+    auto f = pk ? k_gmres_update<float, true> : k_gmres_update<float, false>;
+    hipLaunchKernelGGL(f, dim3(nb), dim3(256), 0, s, n, (float*)x, zl, m, use_vscale, st);
+  }
 }
 
 // ------------------------------------------------------- FGMRESV classical Gram-Schmidt
@@ -203,7 +335,7 @@ void launch_gmres_update(hipStream_t s, long n, int vt, void* x, void* const* zs
 // step from the same w (:238-241) before removing any (:242-244), so the projections need
 // no pass between them: up to 4 come from one read of w.  The basis is stored unscaled,
 // v_k = vs[k] * vscale[k], as in the modified Gram-Schmidt path.
-template <typename VT>
+template <typename VT, bool PK>
 __global__ __launch_bounds__(256) void k_cgs_dots(long n, const VT* __restrict__ w, ZList vl,
                                                   int k0, int nq, const KState* st,
                                                   double* partial) {
@@ -216,24 +348,50 @@ __global__ __launch_bounds__(256) void k_cgs_dots(long n, const VT* __restrict__
     v[q] = q < nq ? (const VT*)vl.z[k0 + q] : w;
   }
   const long stride = (long)gridDim.x * blockDim.x;
-  constexpr int U = 4;  // U grid-stride turns at a time, loads first (same order and sums)
-  for (long e0 = blockIdx.x * (long)blockDim.x + threadIdx.x; e0 < n; e0 += U * stride) {
-    VT wl[U], vl4[4][U];
+  const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if constexpr (PK) {
+    constexpr int W = Pk<VT>::W;
+    const long nc = n / W;
+    for (long c = gid; c < nc; c += stride) {
+      const Pk<VT> wp = ldpk(w, c);
+      Pk<VT> vp[4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const long e = e0 + u * stride;
-      const bool ok = e < n;
-      wl[u] = ok ? w[e] : (VT)0;
+      for (int q = 0; q < 4; ++q) vp[q] = q < nq ? ldpk(v[q], c) : zpk<VT>();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) vl4[q][u] = (ok && q < nq) ? v[q][e] : (VT)0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (e0 + u * stride < n) {
-        const double wv = (double)wl[u];
+      for (int j = 0; j < W; ++j) {
+        const double wv = (double)wp.v[j];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          if (q < nq) acc[q] += ((double)vl4[q][u] * sk[q]) * wv;
+          if (q < nq) acc[q] += ((double)vp[q].v[j] * sk[q]) * wv;
+      }
+    }
+    const long e = nc * W + gid;
+    if (e < n) {
+      const double wv = (double)w[e];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < nq) acc[q] += ((double)v[q][e] * sk[q]) * wv;
+    }
+  } else {
+    constexpr int U = 4;  // U grid-stride turns at a time, loads first (same order and sums)
+    for (long e0 = gid; e0 < n; e0 += U * stride) {
+      VT wl[U], vl4[4][U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long e = e0 + u * stride;
+        const bool ok = e < n;
+        wl[u] = ok ? w[e] : (VT)0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vl4[q][u] = (ok && q < nq) ? v[q][e] : (VT)0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (e0 + u * stride < n) {
+          const double wv = (double)wl[u];
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (q < nq) acc[q] += ((double)vl4[q][u] * sk[q]) * wv;
+        }
       }
     }
   }
@@ -242,7 +400,7 @@ __global__ __launch_bounds__(256) void k_cgs_dots(long n, const VT* __restrict__
 
 // w -= H(k,i) v_k for k = 0..i in that order (the DAXPYs of Iter_Mod.f90:242-244, each
 // rounded to the vector type as a separate update), then |w|^2.
-template <typename VT>
+template <typename VT, bool PK>
 __global__ __launch_bounds__(256) void k_cgs_update(long n, VT* __restrict__ w, ZList vl,
                                                     int i, const KState* st, double* partial) {
   if (st->cycle_done) return;
@@ -254,13 +412,48 @@ __global__ __launch_bounds__(256) void k_cgs_update(long n, VT* __restrict__ w, 
   }
   double acc[1] = {0.0};
   const long stride = (long)gridDim.x * blockDim.x;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n; e += stride) {
-    double wv = (double)w[e];
+  const long gid = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if constexpr (PK) {
+    constexpr int W = Pk<VT>::W;
+    const long nc = n / W;
+    for (long c = gid; c < nc; c += stride) {
+      Pk<VT> wp = ldpk(w, c);
+      double wv[W];
 #pragma unroll
-    for (int k = 0; k < kMaxM; ++k)
-      if (k <= i) wv = (double)(VT)(wv + mh[k] * ((double)((const VT*)vl.z[k])[e] * sk[k]));
-    w[e] = (VT)wv;
-    acc[0] += wv * wv;
+      for (int j = 0; j < W; ++j) wv[j] = (double)wp.v[j];
+#pragma unroll
+      for (int k = 0; k < kMaxM; ++k)
+        if (k <= i) {
+          const Pk<VT> vp = ldpk((const VT*)vl.z[k], c);
+#pragma unroll
+          for (int j = 0; j < W; ++j)
+            wv[j] = (double)(VT)(wv[j] + mh[k] * ((double)vp.v[j] * sk[k]));
+        }
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        wp.v[j] = (VT)wv[j];
+        acc[0] += wv[j] * wv[j];
+      }
+      stpk(w, c, wp);
+    }
+    const long e = nc * W + gid;
+    if (e < n) {
+      double wv = (double)w[e];
+#pragma unroll
+      for (int k = 0; k < kMaxM; ++k)
+        if (k <= i) wv = (double)(VT)(wv + mh[k] * ((double)((const VT*)vl.z[k])[e] * sk[k]));
+      w[e] = (VT)wv;
+      acc[0] += wv * wv;
+    }
+  } else {
+    for (long e = gid; e < n; e += stride) {
+      double wv = (double)w[e];
+#pragma unroll
+      for (int k = 0; k < kMaxM; ++k)
+        if (k <= i) wv = (double)(VT)(wv + mh[k] * ((double)((const VT*)vl.z[k])[e] * sk[k]));
+      w[e] = (VT)wv;
+      acc[0] += wv * wv;
+    }
   }
   block_reduce_store_b<1>(acc, partial, blockIdx.x);
 }
@@ -268,25 +461,39 @@ __global__ __launch_bounds__(256) void k_cgs_update(long n, VT* __restrict__ w, 
 void launch_cgs_dots(hipStream_t s, long n, int vt, const void* w, void* const* vs, int k0,
                      int nq, const KState* st, double* partial) {
   ZList vl;  // vs holds at least k0 + nq vectors
-  for (int q = 0; q < kMaxM; ++q) vl.z[q] = q < k0 + nq ? vs[q] : nullptr;
-  if (vt == 0)
-    hipLaunchKernelGGL(k_cgs_dots<double>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
-                       (const double*)w, vl, k0, nq, st, partial);
-  else
-    hipLaunchKernelGGL(k_cgs_dots<float>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
-                       (const float*)w, vl, k0, nq, st, partial);
+  bool pk = al16(w);
+  for (int q = 0; q < kMaxM; ++q) {
+    vl.z[q] = q < k0 + nq ? vs[q] : nullptr;
+    if (q >= k0 && q < k0 + nq) pk = pk && al16(vs[q]);
+  }
+  if (vt == 0) {
+    auto f = pk ? k_cgs_dots<double, true> : k_cgs_dots<double, false>;
+    hipLaunchKernelGGL(f, dim3(blas_blocks(n)), dim3(256), 0, s, n, (const double*)w, vl, k0,
+                       nq, st, partial);
+  } else {
+    auto f = pk ? k_cgs_dots<float, true> : k_cgs_dots<float, false>;
+    hipLaunchKernelGGL(f, dim3(blas_blocks(n)), dim3(256), 0, s, n, (const float*)w, vl, k0,
+                       nq, st, partial);
+  }
 }
 
 void launch_cgs_update(hipStream_t s, long n, int vt, void* w, void* const* vs, int i,
                        const KState* st, double* partial) {
   ZList vl;  // vs holds at least i + 1 vectors
-  for (int q = 0; q < kMaxM; ++q) vl.z[q] = q <= i ? vs[q] : nullptr;
-  if (vt == 0)
-    hipLaunchKernelGGL(k_cgs_update<double>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
-                       (double*)w, vl, i, st, partial);
-  else
-    hipLaunchKernelGGL(k_cgs_update<float>, dim3(blas_blocks(n)), dim3(256), 0, s, n,
-                       (float*)w, vl, i, st, partial);
+  bool pk = al16(w);
+  for (int q = 0; q < kMaxM; ++q) {
+    vl.z[q] = q <= i ? vs[q] : nullptr;
+    if (q <= i) pk = pk && al16(vs[q]);
+  }
+  if (vt == 0) {
+    auto f = pk ? k_cgs_update<double, true> : k_cgs_update<double, false>;
+    hipLaunchKernelGGL(f, dim3(blas_blocks(n)), dim3(256), 0, s, n, (double*)w, vl, i, st,
+                       partial);
+  } else {
+    auto f = pk ? k_cgs_update<float, true> : k_cgs_update<float, false>;
+    hipLaunchKernelGGL(f, dim3(blas_blocks(n)), dim3(256), 0, s, n, (float*)w, vl, i, st,
+                       partial);
+  }
 }
 
 // ------------------------------------------------------------- generic BLAS-1
