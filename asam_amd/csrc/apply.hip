@@ -26,7 +26,8 @@ static int band_rows(const Geom& g) {
   return rb;
 }
 
-// Variants 0 and 1: tiles of bx x (256/bx) columns by kp planes, at most ~apply_wg
+// Variants 0, 1 and 3: tiles of bx x (256/bx) columns (variant 3: x apply_rows rows per
+// thread) by kp planes, at most ~apply_wg
 // workgroups, padded to a multiple of the 8 XCDs so that the physical -> logical remap below
 // is a bijection.  Variant 2: apply_wg workgroups (a multiple of 8) looping over the tiles
 // of their XCD's bands.
@@ -44,7 +45,8 @@ static ApplyShape apply_shape(const Geom& g) {
     a.nblocks = nb;
     return a;
   }
-  a.gy = (g.ny + byw - 1) / byw;
+  const int rows = g.tune.apply_variant == 3 ? byw * g.tune.apply_rows : byw;
+  a.gy = (g.ny + rows - 1) / rows;
   long cols = (long)a.gx * a.gy;
   long want = g.tune.apply_wg;
   int kp = (int)((cols * g.nz + want - 1) / want);
@@ -291,6 +293,21 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
   }
 }
 
+// Row-blocked z-march variant: R rows per thread (march_rows), tiles of an XCD contiguous.
+template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
+__global__ __launch_bounds__(256) void k_apply_rows(KApply<CT, VX, VO, VU> a) {
+  if (a.done && *a.done) return;
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
+  const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
+  double acc[2] = {0.0, 0.0};
+  march_rows<NP, TR, FL, R, CT, VX, VO, VU>(a, L, acc);
+  if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
+    block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
+    if (a.tail.kind >= 0) tail_arrive(a.tail, tid);
+  }
+}
+
 template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typename VU>
 static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const ApplyReq& r) {
   KApply<CT, VX, VO, VU> a;
@@ -307,6 +324,14 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
     // the x = 0 form too, so that it writes the partial sums of the full form
     hipLaunchKernelGGL((k_apply_band<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
                        dim3(a.bxw, a.byw), 0, s, a);
+  } else if (g.tune.apply_variant == 3) {
+    // the x = 0 form too (same walk, so the same partial sums as the full form)
+    if (g.tune.apply_rows == 2)
+      hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 2, CT, VX, VO, VU>), dim3(sh.nblocks),
+                         dim3(a.bxw, a.byw), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 4, CT, VX, VO, VU>), dim3(sh.nblocks),
+                         dim3(a.bxw, a.byw), 0, s, a);
   } else if constexpr ((FL & AF_X0) != 0) {
     // same grid and partial-sum slots in either variant
     hipLaunchKernelGGL((k_apply_march<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),

@@ -256,4 +256,275 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
   }
 }
 
+// Row-blocked z-march: a thread owns rows j0 .. j0+R-1 of column i of tile L (the tile
+// spans bxw columns by byw*R rows) and walks its planes upwards with all R cells of a level
+// side by side.  A row's south and north neighbours (x, DUT) and its south-face coefficients
+// are the adjacent rows' registers at the same level; only the rows just below and above the
+// block are loaded.  Rows past ny are clamped to row ny-1 (equal to their neighbour, so a
+// wall face difference is exactly zero, as in march_tile) and store nothing.  The arithmetic
+// of a cell is march_tile's, term for term; the dots accumulate level by level, rows in order.
+template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
+__device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int L,
+                                           double (&acc)[2]) {
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long sy = nx, sz = (long)nx * ny;
+  const CT* __restrict__ kx = a.c.kx;
+  const CT* __restrict__ ky = a.c.ky;
+  const CT* __restrict__ kz = a.c.kz;
+  const CT* __restrict__ tx = a.c.tx;
+  const CT* __restrict__ ty = a.c.ty;
+  const CT* __restrict__ tz = a.c.tz;
+  const CT* __restrict__ gzz = a.c.gz;
+  const CT* __restrict__ vol = a.c.vol;
+  const CT* __restrict__ dut = a.c.dut;
+  const VX* __restrict__ X = a.x;
+  const float* __restrict__ DX = a.dx;
+  VO* __restrict__ Y = a.y;
+  VO* __restrict__ Y2 = a.y2;
+  const VU* __restrict__ U = a.u;
+  double* __restrict__ XW = a.xw;
+  const double* __restrict__ Bv = a.b;
+  const CT fa = a.fa;
+  if (L >= a.ntiles) return;
+  const int bx = L % a.gx, rest = L / a.gx, by = rest % a.gy, kt = rest / a.gy;
+  const int i = bx * a.bxw + threadIdx.x;
+  const int j0 = (by * a.byw + threadIdx.y) * R;
+  if (i >= nx || j0 >= ny) return;
+  const int k0 = kt * a.kp, k1 = min(nz, k0 + a.kp);
+  long cb[R];  // cell index of each row at level 0 (rows past ny: row ny-1)
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    ok[r] = j0 + r < ny;
+    cb[r] = i + sy * (ok[r] ? j0 + r : ny - 1);
+  }
+  // the rows just below and above the block (a wall: the block's own edge row)
+  const long cbS = j0 > 0 ? cb[0] - sy : cb[0];
+  const long cbN = j0 + R < ny ? cb[R - 1] + sy : cb[R - 1];
+  if constexpr ((FL & AF_X0) != 0) {
+    static_assert((FL & AF_B) != 0 && (FL & (AF_DX | AF_YU)) == 0, "x = 0 residual form");
+    const unsigned char* __restrict__ D = a.dead;
+    for (int k = k0; k < k1; ++k) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const long c = cb[r] + sz * k;
+        const bool dead = D[c] != 0;
+        if constexpr (NP == 1) {
+          const double yv = dead ? 0.0 : Bv[c];
+          if (ok[r]) {
+            Y[c] = (VO)yv;
+            if constexpr ((FL & AF_Y2) != 0) Y2[c] = (VO)yv;
+            if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
+          }
+        } else {
+          const double2 b0 = *reinterpret_cast<const double2*>(Bv + 2 * c);
+          const double yv1 = dead ? 0.0 : b0.x, yv2 = dead ? 0.0 : b0.y;
+          if (ok[r]) {
+            st2(Y, c, yv1, yv2);
+            if constexpr ((FL & AF_Y2) != 0) st2(Y2, c, yv1, yv2);
+            if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+          }
+        }
+      }
+    }
+    return;
+  }
+  const long dW = i > 0 ? -1 : 0, dE = i + 1 < nx ? 1 : 0;
+  if constexpr (NP == 1) {
+    auto LX = [&](long m) -> CT {
+      CT v = (CT)X[m];
+      if constexpr ((FL & AF_DX) != 0) v += (CT)DX[m];
+      return v;
+    };
+    CT xm[R], dm[R], kzm[R], x0[R], d0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long c = cb[r] + sz * k0;
+      xm[r] = 0; dm[r] = 0; kzm[r] = 0;
+      if (k0 > 0) { xm[r] = LX(c - sz); dm[r] = dut[c - sz]; kzm[r] = kz[c - sz]; }
+      x0[r] = LX(c); d0[r] = dut[c];
+    }
+    for (int k = k0; k < k1; ++k) {
+      const long off = sz * k;
+      const bool up = k + 1 < nz;
+      CT xp[R], dp[R], kz0[R], kyc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const long c = cb[r] + off, mU = up ? c + sz : c;
+        xp[r] = LX(mU); dp[r] = dut[mU]; kz0[r] = kz[c]; kyc[r] = ky[c];
+      }
+      const long cS = cbS + off, cN = cbN + off;
+      const CT xS0 = LX(cS), dS0 = dut[cS], kyS0 = ky[cS];
+      const CT xN1 = LX(cN), dN1 = dut[cN];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const long c = cb[r] + off;
+        const long mW = c + dW, mE = c + dE;
+        const CT xs = r > 0 ? x0[r - 1] : xS0, ds = r > 0 ? d0[r - 1] : dS0;
+        const CT xn = r + 1 < R ? x0[r + 1] : xN1, dn = r + 1 < R ? d0[r + 1] : dN1;
+        const CT kys = r > 0 ? kyc[r - 1] : kyS0;
+        VU u0 = 0;
+        if constexpr ((FL & AF_YU) != 0) u0 = U[c];
+        double b0 = 0.0;
+        if constexpr ((FL & AF_B) != 0) b0 = Bv[c];
+        CT s = fa * vol[c] * x0[r];
+        if constexpr (TR == 0) {
+          const CT q0 = d0[r] * x0[r];
+          s += kx[mW] * (q0 - dut[mW] * LX(mW));
+          s += kx[c] * (q0 - dut[mE] * LX(mE));
+          s += kys * (q0 - ds * xs);
+          s += kyc[r] * (q0 - dn * xn);
+          s += kzm[r] * (q0 - dm[r] * xm[r]);
+          s += kz0[r] * (q0 - dp[r] * xp[r]);
+        } else {
+          CT h = kx[mW] * (x0[r] - LX(mW));
+          h += kx[c] * (x0[r] - LX(mE));
+          h += kys * (x0[r] - xs);
+          h += kyc[r] * (x0[r] - xn);
+          h += kzm[r] * (x0[r] - xm[r]);
+          h += kz0[r] * (x0[r] - xp[r]);
+          s += d0[r] * h;
+        }
+        double yv;
+        if constexpr ((FL & AF_B) != 0) {
+          const bool dead = vol[c] == (CT)0 &&
+                            kx[mW] + kx[c] + kys + kyc[r] + kzm[r] + kz0[r] == (CT)0;
+          yv = dead ? 0.0 : b0 + (double)s;
+        } else {
+          yv = -(double)s;
+        }
+        if (ok[r]) {
+          if constexpr ((FL & AF_DX) != 0) XW[c] = (double)x0[r];
+          Y[c] = (VO)yv;
+          if constexpr ((FL & AF_Y2) != 0) Y2[c] = (VO)yv;
+          if constexpr ((FL & AF_YY) != 0) acc[0] += yv * yv;
+          if constexpr ((FL & AF_YU) != 0) acc[1] += yv * (double)u0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        xm[r] = x0[r]; dm[r] = d0[r]; kzm[r] = kz0[r]; x0[r] = xp[r]; d0[r] = dp[r];
+      }
+    }
+  } else {
+    auto LX = [&](long m) -> V2<CT> {
+      V2<VX> v = ld2(X, m);
+      V2<CT> r{(CT)v.a, (CT)v.b};
+      if constexpr ((FL & AF_DX) != 0) {
+        V2<float> e = ld2(DX, m);
+        r.a += (CT)e.a; r.b += (CT)e.b;
+      }
+      return r;
+    };
+    V2<CT> xm[R], x0[R];
+    CT dm[R], vm[R], kzm[R], tzm[R], gzm[R], d0[R], v0[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long c = cb[r] + sz * k0;
+      xm[r] = V2<CT>{0, 0}; dm[r] = 0; vm[r] = 0; kzm[r] = 0; tzm[r] = 0; gzm[r] = 0;
+      if (k0 > 0) {
+        const long m = c - sz;
+        xm[r] = LX(m); dm[r] = dut[m]; vm[r] = ld_once(vol, m); kzm[r] = ld_once(kz, m);
+        tzm[r] = ld_once(tz, m); gzm[r] = ld_once(gzz, m);
+      }
+      x0[r] = LX(c); d0[r] = dut[c]; v0[r] = ld_once(vol, c);
+    }
+    for (int k = k0; k < k1; ++k) {
+      const long off = sz * k;
+      const bool up = k + 1 < nz;
+      V2<CT> xp[R];
+      CT dp[R], vp[R], kz0[R], tz0[R], gz0[R], kyc[R], tyc[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const long c = cb[r] + off, mU = up ? c + sz : c;
+        xp[r] = LX(mU); dp[r] = dut[mU]; vp[r] = ld_once(vol, mU);
+        kz0[r] = ld_once(kz, c); tz0[r] = ld_once(tz, c); gz0[r] = ld_once(gzz, c);
+        kyc[r] = ky[c]; tyc[r] = ty[c];
+      }
+      const long cS = cbS + off, cN = cbN + off;
+      const V2<CT> xS0 = LX(cS);
+      const CT dS0 = dut[cS], kyS0 = ky[cS], tyS0 = ty[cS];
+      const V2<CT> xN1 = LX(cN);
+      const CT dN1 = dut[cN];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const long c = cb[r] + off;
+        const long mW = c + dW, mE = c + dE;
+        const V2<CT> xW = LX(mW), xE = LX(mE);
+        const V2<CT> xs = r > 0 ? x0[r - 1] : xS0, xn = r + 1 < R ? x0[r + 1] : xN1;
+        const CT ds = r > 0 ? d0[r - 1] : dS0, dn = r + 1 < R ? d0[r + 1] : dN1;
+        const CT kys = r > 0 ? kyc[r - 1] : kyS0, tys = r > 0 ? tyc[r - 1] : tyS0;
+        V2<VU> u0{0, 0};
+        if constexpr ((FL & AF_YU) != 0) u0 = ld2(U, c);
+        double2 b0 = make_double2(0.0, 0.0);
+        if constexpr ((FL & AF_B) != 0) b0 = *reinterpret_cast<const double2*>(Bv + 2 * c);
+        CT s1 = fa * v0[r] * x0[r].a, s2 = v0[r] * x0[r].b;
+        if constexpr (TR == 0) {
+          const CT q0 = d0[r] * x0[r].a;
+          auto hf = [&](CT kk, CT th, CT dnb, CT xnb) {
+            const CT t = kk * (q0 - dnb * xnb);
+            s1 += th * t; s2 += t;
+          };
+          hf(kx[mW], tx[mW], dut[mW], xW.a);
+          hf(kx[c], tx[c], dut[mE], xE.a);
+          hf(kys, tys, ds, xs.a);
+          hf(kyc[r], tyc[r], dn, xn.a);
+          {
+            const CT t = kzm[r] * (q0 - dm[r] * xm[r].a);
+            const CT B = gzm[r] * (vm[r] * xm[r].b + v0[r] * x0[r].b);
+            s1 += tzm[r] * (t + B); s2 += t + B;
+          }
+          {
+            const CT t = kz0[r] * (q0 - dp[r] * xp[r].a);
+            const CT B = gz0[r] * (v0[r] * x0[r].b + vp[r] * xp[r].b);
+            s1 += tz0[r] * (t - B); s2 += t - B;
+          }
+        } else {
+          CT h = 0;
+          const V2<CT> xc = x0[r];
+          auto hf = [&](CT kk, CT th, V2<CT> xnb) {
+            h += kk * ((th * xc.a + xc.b) - (th * xnb.a + xnb.b));
+          };
+          hf(kx[mW], tx[mW], xW);
+          hf(kx[c], tx[c], xE);
+          hf(kys, tys, xs);
+          hf(kyc[r], tyc[r], xn);
+          {
+            const CT zc = tzm[r] * xc.a + xc.b, zn = tzm[r] * xm[r].a + xm[r].b;
+            h += kzm[r] * (zc - zn);
+            s2 += gzm[r] * v0[r] * (zc - zn);
+          }
+          {
+            const CT zc = tz0[r] * xc.a + xc.b, zn = tz0[r] * xp[r].a + xp[r].b;
+            h += kz0[r] * (zc - zn);
+            s2 += gz0[r] * v0[r] * (zn - zc);
+          }
+          s1 += d0[r] * h;
+        }
+        double yv1, yv2;
+        if constexpr ((FL & AF_B) != 0) {
+          const bool dead = v0[r] == (CT)0 &&
+                            kx[mW] + kx[c] + kys + kyc[r] + kzm[r] + kz0[r] == (CT)0;
+          yv1 = dead ? 0.0 : b0.x + (double)s1;
+          yv2 = dead ? 0.0 : b0.y + (double)s2;
+        } else {
+          yv1 = -(double)s1; yv2 = -(double)s2;
+        }
+        if (ok[r]) {
+          if constexpr ((FL & AF_DX) != 0) st2(XW, c, (double)x0[r].a, (double)x0[r].b);
+          st2(Y, c, yv1, yv2);
+          if constexpr ((FL & AF_Y2) != 0) st2(Y2, c, yv1, yv2);
+          if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+          if constexpr ((FL & AF_YU) != 0) acc[1] += yv1 * (double)u0.a + yv2 * (double)u0.b;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        xm[r] = x0[r]; dm[r] = d0[r]; vm[r] = v0[r]; kzm[r] = kz0[r]; tzm[r] = tz0[r];
+        gzm[r] = gz0[r]; x0[r] = xp[r]; d0[r] = dp[r]; v0[r] = vp[r];
+      }
+    }
+  }
+}
+
 }  // namespace asam

@@ -132,12 +132,14 @@ extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning*
   t->apply_bx = c->g.tune.apply_bx;
   t->line_tiled = c->g.tune.line_tiled;
   t->apply_band_rows = c->g.tune.apply_band_rows;
+  t->apply_rows = c->g.tune.apply_rows;
   return 0;
 }
 
 extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning* t) {
   if (!c || !t) return ASAM_E_INVALID;
-  if (t->apply_variant < 0 || t->apply_variant > 2) return fail(c, ASAM_E_INVALID, "apply_variant");
+  if (t->apply_variant < 0 || t->apply_variant > 3) return fail(c, ASAM_E_INVALID, "apply_variant");
+  if (t->apply_rows != 2 && t->apply_rows != 4) return fail(c, ASAM_E_INVALID, "apply_rows");
   if (t->apply_band_rows < 0) return fail(c, ASAM_E_INVALID, "apply_band_rows");
   if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
   if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
@@ -156,6 +158,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.apply_bx = t->apply_bx;
   nt.line_tiled = t->line_tiled ? 1 : 0;
   nt.apply_band_rows = t->apply_band_rows;
+  nt.apply_rows = t->apply_rows;
   Geom tg = c->g;
   tg.tune = nt;
   if (apply_blocks(tg) > kRedBlocks) return fail(c, ASAM_E_INVALID, "apply_wg too large");
@@ -1429,6 +1432,21 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
         if (rc) { c->g.tune = t0; return rc; }
         if (m < best_ms) { best_ms = m; bt = c->g.tune; }
       }
+  // row-blocked z-march: rows per thread, width, XCD order, workgroups
+  const int rbs[] = {2, 4};
+  for (int rr : rbs)
+    for (int bx : bxs)
+      for (int xcd : xcds)
+        for (int wg : wgs) {
+          c->g.tune = bt;
+          c->g.tune.apply_variant = 3; c->g.tune.apply_rows = rr; c->g.tune.apply_bx = bx;
+          c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
+          if (apply_blocks(c->g) > kRedBlocks) continue;
+          double m = 0;
+          int rc = measure(&m);
+          if (rc) { c->g.tune = t0; return rc; }
+          if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+        }
   const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48, 80};
   const Tuning ba = bt;
   for (int w : windows)

@@ -93,7 +93,8 @@ typedef struct {
 typedef struct {
   int apply_variant; /* 0: z-march keeping the column in registers, 1: one cell per thread,
                         2: one cell per thread in band order (each XCD streams its own bands
-                        of rows plane by plane) */
+                        of rows plane by plane), 3: z-march of apply_rows rows of a column
+                        per thread (the inner rows' y-neighbours come from registers) */
   int apply_wg;      /* target workgroups per stencil-apply launch (256..16384; variant 2:
                         the workgroups of the launch, each looping over its XCD's tiles) */
   int line_window;   /* levels per load window of the line solve: 1, 4 or 8 */
@@ -110,6 +111,7 @@ typedef struct {
   int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
   int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
   int apply_band_rows; /* variant 2: rows per band (0: ny/8, one band per XCD) */
+  int apply_rows;    /* variant 3: rows of a column each thread walks together, 2 or 4 */
 } asam_sound_tuning;
 
 /* ---- context ---------------------------------------------------------- */

@@ -401,17 +401,20 @@ def test_moist_setup_matches_oracle(np1, oracle_mod, sound_mod):
 @pytest.mark.parametrize("np1", [1, 2])
 def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
     """Every apply variant (z-march, one cell per thread, band order with one band per XCD
-    or 8-row bands) gives the oracle's MatVec1 to round-off and the FP64 solve to 1e-10,
-    with the residual form of the refinement step and the all-zero-guess form included
-    (mixed solve to 1e-8)."""
+    or 8-row bands, z-march of 2 or 4 rows per thread with row blocks overhanging ny) gives
+    the oracle's MatVec1 to round-off and the FP64 solve to 1e-10, with the residual form of
+    the refinement step and the all-zero-guess form included (mixed solve to 1e-8)."""
     p = small(np1, nx=70, ny=37, nz=11)
     orc = oracle_mod.Oracle(p)
     xo, mo, _, _ = orc.bicgstab(p.b, 300, 1e-8)
     x = live(p, W.normal(p.n, 85))
     s = sound_mod.SoundSolver.from_problem(p)
-    for var, bx, rows, wg in ((0, 256, 0, 4096), (1, 64, 0, 4096), (2, 256, 0, 512),
-                              (2, 64, 8, 1024), (2, 128, 16, 256)):
-        s.set_tuning(apply_variant=var, apply_bx=bx, apply_band_rows=rows, apply_wg=wg)
+    for var, bx, rows, wg, rr in ((0, 256, 0, 4096, 4), (1, 64, 0, 4096, 4),
+                                  (2, 256, 0, 512, 4), (2, 64, 8, 1024, 4),
+                                  (2, 128, 16, 256, 4), (3, 256, 0, 4096, 2),
+                                  (3, 64, 0, 4096, 4), (3, 128, 0, 256, 4)):
+        s.set_tuning(apply_variant=var, apply_bx=bx, apply_band_rows=rows, apply_wg=wg,
+                     apply_rows=rr)
         e = rel(s.MatVec1(x), orc.matvec1(x))
         et = rel(s.MatVec1_T(x), orc.matvec1_T(x))
         s.set_control(precision="FP64")
@@ -419,9 +422,31 @@ def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
         s.set_control(precision="MIXED")
         xm, _, _, im = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
         rr = np.linalg.norm(p.b - orc.matvec1(xm)) / np.linalg.norm(p.b)
-        print(f"np1={np1} variant {var} bx {bx} rows {rows} wg {wg}: MatVec1 {e:.2e} "
+        print(f"np1={np1} variant {var} bx {bx} rows {rows}/{rr} wg {wg}: MatVec1 {e:.2e} "
               f"transposed {et:.2e}, FP64 solve {rel(xg, xo):.2e} MaxIter {mg} (oracle {mo}), "
               f"mixed true residual {rr:.2e}")
         assert e < OP_TOL and et < OP_TOL
         assert rel(xg, xo) < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
         assert im.status == 0 and rr <= 1.0001e-8
+
+
+@pytest.mark.parametrize("np1", [1, 2])
+@pytest.mark.parametrize("dims", [(9, 1, 6), (5, 3, 4), (66, 5, 3)])
+def test_row_blocked_apply_thin_grids(np1, dims, oracle_mod, sound_mod):
+    """The row-blocked z-march on grids thinner than its row blocks (one row, three rows,
+    five rows against blocks of 2 and 4): operator, transposed operator and the residual
+    form against the oracle."""
+    p = small(np1, nx=dims[0], ny=dims[1], nz=dims[2])
+    orc = oracle_mod.Oracle(p)
+    x = live(p, W.normal(p.n, 86))
+    s = sound_mod.SoundSolver.from_problem(p)
+    for bx, rr in ((64, 2), (64, 4), (256, 4)):
+        s.set_tuning(apply_variant=3, apply_bx=bx, apply_rows=rr)
+        e = rel(s.MatVec1(x), orc.matvec1(x))
+        et = rel(s.MatVec1_T(x), orc.matvec1_T(x))
+        print(f"np1={np1} {dims} bx {bx} rows {rr}: MatVec1 {e:.2e} transposed {et:.2e}")
+        assert e < OP_TOL and et < OP_TOL
+        s.set_control(precision="FP64")
+        xg, mg, _, info = s.SolveSound(np.zeros(p.n), p.b, 300, 1e-8)
+        xo, mo, _, _ = orc.bicgstab(p.b, 300, 1e-8)
+        assert rel(xg, xo) < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
