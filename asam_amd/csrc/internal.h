@@ -35,10 +35,10 @@ struct Coef {
 
 // Launch-shape knobs (set by the autotuner or asam_sound_set_tuning).
 struct Tuning {
-  int apply_variant = 0;  // 0: z-march (column kept in registers), 1: one cell per thread,
+  int apply_variant = 3;  // 0: z-march (column kept in registers), 1: one cell per thread,
                           // 2: one cell per thread in band order (each XCD its own bands),
                           // 3: z-march of apply_rows rows per thread
-  int apply_wg = 4096;    // target workgroups per apply launch
+  int apply_wg = 2048;    // target workgroups per apply launch
   int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
   int line_lds = 1;       // LDS staging budget of the line solve's forward sweep per
                           // workgroup: 0 none (output vector), 1 default, n >= 2 n KiB
@@ -46,7 +46,7 @@ struct Tuning {
   int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
   int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
-  int apply_rows = 4;     // variant 3: rows per thread (2 or 4)
+  int apply_rows = 2;     // variant 3: rows per thread (2 or 4)
   int fused_tail = 0;     // 1: BiCGStab and refinement tails run by the producing kernel's
                           // last workgroup (0: a separate finalize launch each)
 };
