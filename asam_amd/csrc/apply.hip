@@ -8,7 +8,7 @@
 // columns per wavefront (coalesced along x at every level), loads issued a window of
 // levels ahead of the recurrence, forward-sweep results staged in LDS for the
 // backward sweep.
-#include "../../include/asam_sound.h"
+#include "asam_sound.h"
 #include "apply_tile.h"
 
 namespace asam {
@@ -31,7 +31,7 @@ static int band_rows(const Geom& g) {
 // workgroups, padded to a multiple of the 8 XCDs so that the physical -> logical remap below
 // is a bijection.  Variant 2: apply_wg workgroups (a multiple of 8) looping over the tiles
 // of their XCD's bands.
-static ApplyShape apply_shape(const Geom& g) {
+static ApplyShape apply_shape(const Geom& g, int fp32) {
   ApplyShape a;
   const int bxw = g.tune.apply_bx, byw = 256 / bxw;
   a.gx = (g.nx + bxw - 1) / bxw;
@@ -45,7 +45,9 @@ static ApplyShape apply_shape(const Geom& g) {
     a.nblocks = nb;
     return a;
   }
-  const int rows = g.tune.apply_variant == 3 ? byw * g.tune.apply_rows : byw;
+  // variant 3 tiles: four rows per thread with FP32 coefficients only (see apply_t)
+  const int rpt = (g.tune.apply_variant == 3 && g.tune.apply_rows == 4 && fp32) ? 4 : 2;
+  const int rows = g.tune.apply_variant == 3 ? byw * rpt : byw;
   a.gy = (g.ny + rows - 1) / rows;
   long cols = (long)a.gx * a.gy;
   long want = g.tune.apply_wg;
@@ -59,7 +61,7 @@ static ApplyShape apply_shape(const Geom& g) {
   return a;
 }
 
-int apply_blocks(const Geom& g) { return apply_shape(g).nblocks; }
+int apply_blocks(const Geom& g, int fp32) { return apply_shape(g, fp32).nblocks; }
 
 // The operator at one cell (i, j, k) with the fused options (k_apply and k_apply_band).
 // Neighbour indices are clamped to the cell itself at walls: the face difference
@@ -312,7 +314,7 @@ template <int NP, int TR, int FL, typename CT, typename VX, typename VO, typenam
 static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const ApplyReq& r) {
   KApply<CT, VX, VO, VU> a;
   a.c = c; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.fa = (CT)g.fa;
-  ApplyShape sh = apply_shape(g);
+  ApplyShape sh = apply_shape(g, sizeof(CT) == 4 ? 1 : 0);
   a.gx = sh.gx; a.gy = sh.gy; a.kp = sh.kp; a.ntiles = sh.ntiles;
   a.bxw = g.tune.apply_bx; a.byw = 256 / g.tune.apply_bx; a.xcd = g.tune.apply_xcd;
   a.x = (const VX*)r.x; a.dx = r.dx; a.xw = r.xw; a.b = r.b; a.y = (VO*)r.y;
@@ -325,13 +327,19 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
     hipLaunchKernelGGL((k_apply_band<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),
                        dim3(a.bxw, a.byw), 0, s, a);
   } else if (g.tune.apply_variant == 3) {
-    // the x = 0 form too (same walk, so the same partial sums as the full form)
-    if (g.tune.apply_rows == 2)
+    // the x = 0 form too (same walk, so the same partial sums as the full form).  Four rows
+    // only with FP32 coefficients: FP64 kernels of four rows spill registers, and the FP64
+    // applies of a mixed solve share the FP32 inner solve's tuning.
+    bool four = false;
+    if constexpr (sizeof(CT) == 4) four = g.tune.apply_rows == 4;
+    if (four) {
+      if constexpr (sizeof(CT) == 4)
+        hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 4, CT, VX, VO, VU>), dim3(sh.nblocks),
+                           dim3(a.bxw, a.byw), 0, s, a);
+    } else {
       hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 2, CT, VX, VO, VU>), dim3(sh.nblocks),
                          dim3(a.bxw, a.byw), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 4, CT, VX, VO, VU>), dim3(sh.nblocks),
-                         dim3(a.bxw, a.byw), 0, s, a);
+    }
   } else if constexpr ((FL & AF_X0) != 0) {
     // same grid and partial-sum slots in either variant
     hipLaunchKernelGGL((k_apply_march<NP, TR, FL, CT, VX, VO, VU>), dim3(sh.nblocks),

@@ -263,6 +263,10 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
 // block are loaded.  Rows past ny are clamped to row ny-1 (equal to their neighbour, so a
 // wall face difference is exactly zero, as in march_tile) and store nothing.  The arithmetic
 // of a cell is march_tile's, term for term; the dots accumulate level by level, rows in order.
+// The x and DUT of the level above at the west and east neighbours and in the rows just
+// below and above the block are loaded in the iteration that loads the cell above (the
+// owners of those cells load them then too) and carried one level, so they are not
+// re-read from the caches an iteration later.
 template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
 __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int L,
                                            double (&acc)[2]) {
@@ -336,26 +340,34 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
       if constexpr ((FL & AF_DX) != 0) v += (CT)DX[m];
       return v;
     };
-    CT xm[R], dm[R], kzm[R], x0[R], d0[R];
+    CT xm[R], dm[R], kzm[R], x0[R], d0[R], xw0[R], xe0[R], dw0[R], de0[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const long c = cb[r] + sz * k0;
       xm[r] = 0; dm[r] = 0; kzm[r] = 0;
       if (k0 > 0) { xm[r] = LX(c - sz); dm[r] = dut[c - sz]; kzm[r] = kz[c - sz]; }
       x0[r] = LX(c); d0[r] = dut[c];
+      xw0[r] = LX(c + dW); xe0[r] = LX(c + dE); dw0[r] = 0; de0[r] = 0;
+      if constexpr (TR == 0) { dw0[r] = dut[c + dW]; de0[r] = dut[c + dE]; }
     }
+    CT xS0 = LX(cbS + sz * k0), dS0 = 0, xN1 = LX(cbN + sz * k0), dN1 = 0;
+    if constexpr (TR == 0) { dS0 = dut[cbS + sz * k0]; dN1 = dut[cbN + sz * k0]; }
     for (int k = k0; k < k1; ++k) {
       const long off = sz * k;
       const bool up = k + 1 < nz;
-      CT xp[R], dp[R], kz0[R], kyc[R];
+      const long offU = up ? off + sz : off;
+      const CT xSp = LX(cbS + offU), xNp = LX(cbN + offU);
+      CT dSp = 0, dNp = 0;
+      if constexpr (TR == 0) { dSp = dut[cbS + offU]; dNp = dut[cbN + offU]; }
+      CT xp[R], dp[R], kz0[R], kyc[R], xwp[R], xep[R], dwp[R], dep[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long c = cb[r] + off, mU = up ? c + sz : c;
         xp[r] = LX(mU); dp[r] = dut[mU]; kz0[r] = kz[c]; kyc[r] = ky[c];
+        xwp[r] = LX(mU + dW); xep[r] = LX(mU + dE); dwp[r] = 0; dep[r] = 0;
+        if constexpr (TR == 0) { dwp[r] = dut[mU + dW]; dep[r] = dut[mU + dE]; }
       }
-      const long cS = cbS + off, cN = cbN + off;
-      const CT xS0 = LX(cS), dS0 = dut[cS], kyS0 = ky[cS];
-      const CT xN1 = LX(cN), dN1 = dut[cN];
+      const CT kyS0 = ky[cbS + off];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long c = cb[r] + off;
@@ -370,15 +382,15 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
         CT s = fa * vol[c] * x0[r];
         if constexpr (TR == 0) {
           const CT q0 = d0[r] * x0[r];
-          s += kx[mW] * (q0 - dut[mW] * LX(mW));
-          s += kx[c] * (q0 - dut[mE] * LX(mE));
+          s += kx[mW] * (q0 - dw0[r] * xw0[r]);
+          s += kx[c] * (q0 - de0[r] * xe0[r]);
           s += kys * (q0 - ds * xs);
           s += kyc[r] * (q0 - dn * xn);
           s += kzm[r] * (q0 - dm[r] * xm[r]);
           s += kz0[r] * (q0 - dp[r] * xp[r]);
         } else {
-          CT h = kx[mW] * (x0[r] - LX(mW));
-          h += kx[c] * (x0[r] - LX(mE));
+          CT h = kx[mW] * (x0[r] - xw0[r]);
+          h += kx[c] * (x0[r] - xe0[r]);
           h += kys * (x0[r] - xs);
           h += kyc[r] * (x0[r] - xn);
           h += kzm[r] * (x0[r] - xm[r]);
@@ -404,7 +416,9 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         xm[r] = x0[r]; dm[r] = d0[r]; kzm[r] = kz0[r]; x0[r] = xp[r]; d0[r] = dp[r];
+        xw0[r] = xwp[r]; xe0[r] = xep[r]; dw0[r] = dwp[r]; de0[r] = dep[r];
       }
+      xS0 = xSp; dS0 = dSp; xN1 = xNp; dN1 = dNp;
     }
   } else {
     auto LX = [&](long m) -> V2<CT> {
@@ -416,8 +430,8 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
       }
       return r;
     };
-    V2<CT> xm[R], x0[R];
-    CT dm[R], vm[R], kzm[R], tzm[R], gzm[R], d0[R], v0[R];
+    V2<CT> xm[R], x0[R], xw0[R], xe0[R];
+    CT dm[R], vm[R], kzm[R], tzm[R], gzm[R], d0[R], v0[R], dw0[R], de0[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const long c = cb[r] + sz * k0;
@@ -428,29 +442,37 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
         tzm[r] = ld_once(tz, m); gzm[r] = ld_once(gzz, m);
       }
       x0[r] = LX(c); d0[r] = dut[c]; v0[r] = ld_once(vol, c);
+      xw0[r] = LX(c + dW); xe0[r] = LX(c + dE); dw0[r] = 0; de0[r] = 0;
+      if constexpr (TR == 0) { dw0[r] = dut[c + dW]; de0[r] = dut[c + dE]; }
     }
+    V2<CT> xS0 = LX(cbS + sz * k0), xN1 = LX(cbN + sz * k0);
+    CT dS0 = 0, dN1 = 0;
+    if constexpr (TR == 0) { dS0 = dut[cbS + sz * k0]; dN1 = dut[cbN + sz * k0]; }
     for (int k = k0; k < k1; ++k) {
       const long off = sz * k;
       const bool up = k + 1 < nz;
-      V2<CT> xp[R];
-      CT dp[R], vp[R], kz0[R], tz0[R], gz0[R], kyc[R], tyc[R];
+      const long offU = up ? off + sz : off;
+      const V2<CT> xSp = LX(cbS + offU), xNp = LX(cbN + offU);
+      CT dSp = 0, dNp = 0;
+      if constexpr (TR == 0) { dSp = dut[cbS + offU]; dNp = dut[cbN + offU]; }
+      V2<CT> xp[R], xwp[R], xep[R];
+      CT dp[R], vp[R], kz0[R], tz0[R], gz0[R], kyc[R], tyc[R], dwp[R], dep[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long c = cb[r] + off, mU = up ? c + sz : c;
         xp[r] = LX(mU); dp[r] = dut[mU]; vp[r] = ld_once(vol, mU);
         kz0[r] = ld_once(kz, c); tz0[r] = ld_once(tz, c); gz0[r] = ld_once(gzz, c);
         kyc[r] = ky[c]; tyc[r] = ty[c];
+        xwp[r] = LX(mU + dW); xep[r] = LX(mU + dE); dwp[r] = 0; dep[r] = 0;
+        if constexpr (TR == 0) { dwp[r] = dut[mU + dW]; dep[r] = dut[mU + dE]; }
       }
-      const long cS = cbS + off, cN = cbN + off;
-      const V2<CT> xS0 = LX(cS);
-      const CT dS0 = dut[cS], kyS0 = ky[cS], tyS0 = ty[cS];
-      const V2<CT> xN1 = LX(cN);
-      const CT dN1 = dut[cN];
+      const long cS = cbS + off;
+      const CT kyS0 = ky[cS], tyS0 = ty[cS];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long c = cb[r] + off;
         const long mW = c + dW, mE = c + dE;
-        const V2<CT> xW = LX(mW), xE = LX(mE);
+        const V2<CT> xW = xw0[r], xE = xe0[r];
         const V2<CT> xs = r > 0 ? x0[r - 1] : xS0, xn = r + 1 < R ? x0[r + 1] : xN1;
         const CT ds = r > 0 ? d0[r - 1] : dS0, dn = r + 1 < R ? d0[r + 1] : dN1;
         const CT kys = r > 0 ? kyc[r - 1] : kyS0, tys = r > 0 ? tyc[r - 1] : tyS0;
@@ -465,8 +487,8 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
             const CT t = kk * (q0 - dnb * xnb);
             s1 += th * t; s2 += t;
           };
-          hf(kx[mW], tx[mW], dut[mW], xW.a);
-          hf(kx[c], tx[c], dut[mE], xE.a);
+          hf(kx[mW], tx[mW], dw0[r], xW.a);
+          hf(kx[c], tx[c], de0[r], xE.a);
           hf(kys, tys, ds, xs.a);
           hf(kyc[r], tyc[r], dn, xn.a);
           {
@@ -522,7 +544,9 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
       for (int r = 0; r < R; ++r) {
         xm[r] = x0[r]; dm[r] = d0[r]; vm[r] = v0[r]; kzm[r] = kz0[r]; tzm[r] = tz0[r];
         gzm[r] = gz0[r]; x0[r] = xp[r]; d0[r] = dp[r]; v0[r] = vp[r];
+        xw0[r] = xwp[r]; xe0[r] = xep[r]; dw0[r] = dwp[r]; de0[r] = dep[r];
       }
+      xS0 = xSp; dS0 = dSp; xN1 = xNp; dN1 = dNp;
     }
   }
 }

@@ -43,7 +43,7 @@ struct Tuning {
   int line_lds = 1;       // LDS staging budget of the line solve's forward sweep per
                           // workgroup: 0 none (output vector), 1 default, n >= 2 n KiB
   int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
-  int apply_bx = 256;     // apply workgroup width along x (64, 128 or 256; 256 threads)
+  int apply_bx = 256;     // apply workgroup width along x (32, 64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
   int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
   int apply_rows = 2;     // variant 3: rows per thread (2 or 4)
@@ -146,7 +146,8 @@ struct ApplyReq {
   const unsigned char* x0_dead = nullptr;
   KTail tail;                  // fused scalar tail of the dots (needs dots)
 };
-int apply_blocks(const Geom& g);
+// Workgroups (= partial sums) of an apply launch; fp32: FP32 coefficients.
+int apply_blocks(const Geom& g, int fp32);
 // dead[c] = 1 where the residual form writes 0 (no volume and no face coefficient), from
 // the FP64 coefficients with the apply's own test.
 void launch_dead_mask(hipStream_t s, const Geom& g, const Coef<double>& c64, unsigned char* dead);

@@ -152,7 +152,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.line_lds = t->line_lds;
   if (t->fused_tail != 0 && t->fused_tail != 1) return fail(c, ASAM_E_INVALID, "fused_tail");
   nt.fused_tail = t->fused_tail;
-  if (t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
+  if (t->apply_bx != 32 && t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
     return fail(c, ASAM_E_INVALID, "apply_bx");
   nt.apply_xcd = t->apply_xcd ? 1 : 0;
   nt.apply_bx = t->apply_bx;
@@ -161,7 +161,8 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.apply_rows = t->apply_rows;
   Geom tg = c->g;
   tg.tune = nt;
-  if (apply_blocks(tg) > kRedBlocks) return fail(c, ASAM_E_INVALID, "apply_wg too large");
+  if (apply_blocks(tg, 0) > kRedBlocks || apply_blocks(tg, 1) > kRedBlocks)
+    return fail(c, ASAM_E_INVALID, "apply_wg too large");
   const bool relayout = nt.line_tiled != c->g.tune.line_tiled;
   c->g.tune = nt;
   if (relayout && c->ready) {  // the line factors live in the layout the tuning names
@@ -190,7 +191,8 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
     Geom tg;
     tg.nx = grid->nx; tg.ny = grid->ny; tg.nz = grid->nz; tg.np1 = grid->np1; tg.N = N;
     tg.fa = 1.0;
-    if (apply_blocks(tg) > kRedBlocks || line_blocks(tg) > kRedBlocks ||
+    if (apply_blocks(tg, 0) > kRedBlocks || apply_blocks(tg, 1) > kRedBlocks ||
+        line_blocks(tg) > kRedBlocks ||
         (long)((grid->nx + 63) / 64) * ((grid->ny + 3) / 4) > kRedBlocks)
       return ASAM_E_INVALID;
   }
@@ -464,12 +466,12 @@ struct Enq {
     r.y2 = y2; r.u = u; r.dots = dots; r.partial = c->partial; r.done = done; r.dx = dx;
     r.xw = xw;
     const bool fuse = fkind >= 0 && dots && fused(fkind);
-    if (fuse) r.tail = tail(fkind, fnq, apply_blocks(c->g), fnblk_hi);
+    if (fuse) r.tail = tail(fkind, fnq, apply_blocks(c->g, prec), fnblk_hi);
     timed(prec ? 1 : 0, apply_bytes(c->g, r),
           [&] { c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0; },
           tag);
     c->counts[0]++;
-    if (fkind >= 0 && !fuse) fin(fkind, fnq, apply_blocks(c->g), 0, 0, tag, fnblk_hi);
+    if (fkind >= 0 && !fuse) fin(fkind, fnq, apply_blocks(c->g, prec), 0, 0, tag, fnblk_hi);
   }
   // Initial residual y = y2 = b - Op(x), y.y into the partial sums, then finalize fkind.
   // Unprofiled, a scan of x picks on the device which of two launches writes y: the full
@@ -490,12 +492,12 @@ struct Enq {
     // exactly one of the two launches runs (the other returns on its flag), so at most
     // one of them arrives at the fused tail
     const bool fuse = fkind >= 0 && dots && fused(fkind);
-    if (fuse) r.tail = tail(fkind, 1, apply_blocks(c->g), 0);
+    if (fuse) r.tail = tail(fkind, 1, apply_blocks(c->g, 0), 0);
     c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     r.x = nullptr; r.x0_dead = c->dead; r.done = &c->st->xnz;
     c->launch_err |= launch_apply(s, c->g, c->c64, c->c32, r) != 0;
     c->counts[0]++;
-    if (fkind >= 0 && !fuse) fin(fkind, 1, apply_blocks(c->g));
+    if (fkind >= 0 && !fuse) fin(fkind, 1, apply_blocks(c->g, 0));
   }
   void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
             const double* scale, const int* done, int qslot = 0) {
@@ -753,7 +755,7 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classic
         e.resid_init(0, x, c->gv[0], b, nullptr, DOT_YY, -1);
       else
         e.apply(0, 0, 0, 0, x, c->gv[0], b, nullptr, nullptr, DOT_YY, &st->done);
-      e.fin(FIN_GM_INIT, 1, apply_blocks(g));
+      e.fin(FIN_GM_INIT, 1, apply_blocks(g, 0));
     } else {
       if (cyc == 0)
         e.resid_init(0, x, c->gtmp, b, nullptr, 0, -1);
@@ -769,7 +771,7 @@ static int solve_gmres_fp64(asam_sound_ctx* c, int tr, int flexible, int classic
                &st->cycle_done);
         e.apply(0, 0, 0, 0, c->gz[i], c->gv[i + 1], nullptr, nullptr, c->gv[0], DOT_YU,
                 &st->cycle_done);
-        e.fin(FIN_GM_H, 2, apply_blocks(g), 0, i);
+        e.fin(FIN_GM_H, 2, apply_blocks(g, 0), 0, i);
       } else {
         e.apply(0, 0, 0, 0, c->gv[i], c->gtmp, nullptr, nullptr, nullptr, 0, &st->cycle_done);
         e.line(0, LINE_PLAIN, c->gtmp, nullptr, nullptr, c->gv[i + 1], &st->vscale[i],
@@ -947,7 +949,7 @@ static int solve_mixed_gm(asam_sound_ctx* c, int tr, int classical, double* x, c
              &st->cycle_done);
       e.apply(1, 1, 1, 1, c->gz32[i], c->gv32[i + 1], nullptr, nullptr, v0, DOT_YU,
               &st->cycle_done);
-      e.fin(FIN_GM_H, 2, apply_blocks(g), 0, i);
+      e.fin(FIN_GM_H, 2, apply_blocks(g, 1), 0, i);
       if (classical) {
         void* const* vs = (void* const*)c->gv32.data();
         for (int k0 = 1; k0 <= i; k0 += 4)
@@ -1413,7 +1415,7 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
           c->g.tune = bt;
           c->g.tune.apply_variant = var; c->g.tune.apply_bx = bx;
           c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
-          if (apply_blocks(c->g) > kRedBlocks) continue;
+          if (apply_blocks(c->g, 0) > kRedBlocks || apply_blocks(c->g, 1) > kRedBlocks) continue;
           double m = 0;
           int rc = measure(&m);
           if (rc) { c->g.tune = t0; return rc; }
@@ -1433,15 +1435,17 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
         if (m < best_ms) { best_ms = m; bt = c->g.tune; }
       }
   // row-blocked z-march: rows per thread, width, XCD order, workgroups
-  const int rbs[] = {2, 4};
+  // width 32: a wavefront spans two stacked row blocks, so the rows between them are
+  // loaded by one wavefront in one iteration
+  const int rbs[] = {2, 4}, rbxs[] = {32, 64, 128, 256};
   for (int rr : rbs)
-    for (int bx : bxs)
+    for (int bx : rbxs)
       for (int xcd : xcds)
         for (int wg : wgs) {
           c->g.tune = bt;
           c->g.tune.apply_variant = 3; c->g.tune.apply_rows = rr; c->g.tune.apply_bx = bx;
           c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
-          if (apply_blocks(c->g) > kRedBlocks) continue;
+          if (apply_blocks(c->g, 0) > kRedBlocks || apply_blocks(c->g, 1) > kRedBlocks) continue;
           double m = 0;
           int rc = measure(&m);
           if (rc) { c->g.tune = t0; return rc; }

@@ -31,12 +31,13 @@ def kernel_class(name: str):
     """Live-profiler class of a kernel name (None: not a solver kernel)."""
     if "k_copy16" in name:
         return "copy"
-    m = re.search(r"k_(apply_march|apply|line)<[^>]*?(double|float)", name)
+    m = re.search(r"k_(apply\w*|line)<[^>]*?(double|float)", name)
     if m:
         kind = "apply" if m.group(1).startswith("apply") else "line"
         # the first floating type in the template list is the coefficient type
         return f"{kind}_{'fp64' if m.group(2) == 'double' else 'fp32'}"
-    m = re.search(r"k_(bicg_update|mgs|gmres_update)<(double|float)>", name)
+    m = re.search(r"k_(bicg_update|mgs|gmres_update|cgs_dots|cgs_update)<(double|float)[,>]",
+                  name)
     if m:
         return f"blas_{'fp64' if m.group(2) == 'double' else 'fp32'}"
     if "k_finalize" in name:

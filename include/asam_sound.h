@@ -108,7 +108,8 @@ typedef struct {
                         way; the autotuner times both.  (F)GMRES tails are always
                         separate launches */
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
-  int apply_bx;      /* apply workgroup width along x: 64, 128 or 256 */
+  int apply_bx;      /* apply workgroup width along x: 32, 64, 128 or 256 (256 threads; at 32
+                        a wavefront covers two stacked row blocks of variant 3) */
   int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
   int apply_band_rows; /* variant 2: rows per band (0: ny/8, one band per XCD) */
   int apply_rows;    /* variant 3: rows of a column each thread walks together, 2 or 4 */

@@ -412,7 +412,8 @@ def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
     for var, bx, rows, wg, rr in ((0, 256, 0, 4096, 4), (1, 64, 0, 4096, 4),
                                   (2, 256, 0, 512, 4), (2, 64, 8, 1024, 4),
                                   (2, 128, 16, 256, 4), (3, 256, 0, 4096, 2),
-                                  (3, 64, 0, 4096, 4), (3, 128, 0, 256, 4)):
+                                  (3, 64, 0, 4096, 4), (3, 128, 0, 256, 4),
+                                  (3, 32, 0, 2048, 2), (3, 32, 0, 4096, 4)):
         s.set_tuning(apply_variant=var, apply_bx=bx, apply_band_rows=rows, apply_wg=wg,
                      apply_rows=rr)
         e = rel(s.MatVec1(x), orc.matvec1(x))
@@ -434,13 +435,13 @@ def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
 @pytest.mark.parametrize("dims", [(9, 1, 6), (5, 3, 4), (66, 5, 3)])
 def test_row_blocked_apply_thin_grids(np1, dims, oracle_mod, sound_mod):
     """The row-blocked z-march on grids thinner than its row blocks (one row, three rows,
-    five rows against blocks of 2 and 4): operator, transposed operator and the residual
-    form against the oracle."""
+    five rows against blocks of 2 and 4, widths 32 to 256): operator, transposed operator
+    and the residual form against the oracle."""
     p = small(np1, nx=dims[0], ny=dims[1], nz=dims[2])
     orc = oracle_mod.Oracle(p)
     x = live(p, W.normal(p.n, 86))
     s = sound_mod.SoundSolver.from_problem(p)
-    for bx, rr in ((64, 2), (64, 4), (256, 4)):
+    for bx, rr in ((32, 2), (32, 4), (64, 2), (64, 4), (256, 4)):
         s.set_tuning(apply_variant=3, apply_bx=bx, apply_rows=rr)
         e = rel(s.MatVec1(x), orc.matvec1(x))
         et = rel(s.MatVec1_T(x), orc.matvec1_T(x))
