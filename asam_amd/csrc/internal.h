@@ -38,12 +38,12 @@ struct Tuning {
   int apply_variant = 3;  // 0: z-march (column kept in registers), 1: one cell per thread,
                           // 2: one cell per thread in band order (each XCD its own bands),
                           // 3: z-march of apply_rows rows per thread
-  int apply_wg = 2048;    // target workgroups per apply launch
+  int apply_wg = 512;     // target workgroups per apply launch
   int line_window = 0;    // levels per load window of the line solve (1, 4, 8; 0: by type)
   int line_lds = 1;       // LDS staging budget of the line solve's forward sweep per
                           // workgroup: 0 none (output vector), 1 default, n >= 2 n KiB
   int apply_xcd = 1;      // remap workgroups so each XCD gets a contiguous range of tiles
-  int apply_bx = 256;     // apply workgroup width along x (32, 64, 128 or 256; 256 threads)
+  int apply_bx = 128;     // apply workgroup width along x (32, 64, 128 or 256; 256 threads)
   int line_tiled = 1;     // line factors stored column-tiled (64 columns x nz levels)
   int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
   int apply_rows = 2;     // variant 3: rows per thread (2 or 4)

@@ -1437,11 +1437,13 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
   // row-blocked z-march: rows per thread, width, XCD order, workgroups
   // width 32: a wavefront spans two stacked row blocks, so the rows between them are
   // loaded by one wavefront in one iteration
-  const int rbs[] = {2, 4}, rbxs[] = {32, 64, 128, 256};
+  // (fewer workgroups than the other variants: longer z-marches per tile; the best count
+  // depends on the grid and is not monotonic)
+  const int rbs[] = {2, 4}, rbxs[] = {32, 64, 128, 256}, rwgs[] = {256, 512, 1024, 2048, 4096};
   for (int rr : rbs)
     for (int bx : rbxs)
       for (int xcd : xcds)
-        for (int wg : wgs) {
+        for (int wg : rwgs) {
           c->g.tune = bt;
           c->g.tune.apply_variant = 3; c->g.tune.apply_rows = rr; c->g.tune.apply_bx = bx;
           c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
