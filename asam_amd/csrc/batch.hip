@@ -25,7 +25,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/asam_ensemble.h"
+#include "asam_ensemble.h"
 #include "apply_tile.h"
 #include "line_tile.h"
 

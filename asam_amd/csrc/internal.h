@@ -49,7 +49,16 @@ struct Tuning {
   int apply_rows = 2;     // variant 3: rows per thread (2 or 4)
   int fused_tail = 0;     // 1: BiCGStab and refinement tails run by the producing kernel's
                           // last workgroup (0: a separate finalize launch each)
+  int line_nt = 2;        // line solve's factor loads (and the vectors it reads once):
+                          // 0 plain, 1 nontemporal, 2 nontemporal when the factors of the
+                          // solve's precision exceed kLineNtBytes
 };
+
+// Factor footprint above which the line solve's loads default to nontemporal: below it the
+// factors stay resident in the 256 MB Infinity Cache across solves (measured: FP32 MultiTriT
+// 256x256x64, 50 MB of factors, line 24.8 us plain against 28.6 us nontemporal; FP32
+// MultiTriTB, 201 MB, 64.5 us plain against 58.7 us nontemporal).
+constexpr long kLineNtBytes = 64l << 20;
 
 struct Geom {
   int nx, ny, nz, np1;

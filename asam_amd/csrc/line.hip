@@ -46,34 +46,51 @@ int lds_levels(int nz, int np, size_t word, size_t budget) {
 }
 
 // One 64-column tile per workgroup: grid (gxl, ny).
-template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT>
+template <int NP, int TR, int MODE, int LDS, int P, int NT, typename CT, typename VT>
 __global__ __launch_bounds__(64) void k_line(KLine<CT, VT> a) {
   if (a.done && *a.done) return;
   extern __shared__ unsigned char lds_raw[];
   CT* stage = reinterpret_cast<CT*>(lds_raw);
   const int blk = blockIdx.x + gridDim.x * blockIdx.y;
   double acc[1] = {0.0};
-  line_tile<NP, TR, MODE, LDS, P, CT, VT>(a, blockIdx.x, blockIdx.y, gridDim.x, stage, acc);
+  line_tile<NP, TR, MODE, LDS, P, CT, VT, NT>(a, blockIdx.x, blockIdx.y, gridDim.x, stage, acc);
   if constexpr (MODE == LINE_BICG_S) {
     block_reduce_store<1>(acc, a.partial + (long)a.qslot * kRedBlocks, blk, threadIdx.x, 64);
   }
 }
 
-template <int NP, int TR, int LDS, int P, typename CT, typename VT>
-static void line_launch(hipStream_t s, const Geom& g, const KLine<CT, VT>& a, int mode,
-                        size_t lds) {
+template <int NP, int TR, int LDS, int P, int NT, typename CT, typename VT>
+static void line_mode(hipStream_t s, const Geom& g, const KLine<CT, VT>& a, int mode,
+                      size_t lds) {
   dim3 grid((g.nx + 63) / 64, g.ny);
   switch (mode) {
     case LINE_PLAIN:
-      hipLaunchKernelGGL((k_line<NP, TR, LINE_PLAIN, LDS, P, CT, VT>), grid, dim3(64), lds, s, a);
+      hipLaunchKernelGGL((k_line<NP, TR, LINE_PLAIN, LDS, P, NT, CT, VT>), grid, dim3(64), lds, s,
+                         a);
       break;
     case LINE_BICG_P:
-      hipLaunchKernelGGL((k_line<NP, TR, LINE_BICG_P, LDS, P, CT, VT>), grid, dim3(64), lds, s, a);
+      hipLaunchKernelGGL((k_line<NP, TR, LINE_BICG_P, LDS, P, NT, CT, VT>), grid, dim3(64), lds,
+                         s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_line<NP, TR, LINE_BICG_S, LDS, P, CT, VT>), grid, dim3(64), lds, s, a);
+      hipLaunchKernelGGL((k_line<NP, TR, LINE_BICG_S, LDS, P, NT, CT, VT>), grid, dim3(64), lds,
+                         s, a);
       break;
   }
+}
+
+// Cache policy of the factor loads: the tuning's, or by the factors' footprint.
+template <int NP, typename CT>
+static bool line_nontemporal(const Geom& g) {
+  if (g.tune.line_nt != 2) return g.tune.line_nt == 1;
+  return 3l * NP * NP * fac_cells(g) * (long)sizeof(CT) > kLineNtBytes;
+}
+
+template <int NP, int TR, int LDS, int P, typename CT, typename VT>
+static void line_launch(hipStream_t s, const Geom& g, const KLine<CT, VT>& a, int mode,
+                        size_t lds) {
+  if (line_nontemporal<NP, CT>(g)) line_mode<NP, TR, LDS, P, 1, CT, VT>(s, g, a, mode, lds);
+  else line_mode<NP, TR, LDS, P, 0, CT, VT>(s, g, a, mode, lds);
 }
 
 template <int NP, int TR, typename CT, typename VT>

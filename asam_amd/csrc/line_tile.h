@@ -25,15 +25,30 @@ struct KLine {
 
 template <typename T> struct M4 { T m[4]; };
 
-template <typename T>
+// Factor entries are read once per sweep and by no other kernel.  NT = 1 loads them
+// nontemporal, so that factors larger than the Infinity Cache do not push out the vectors
+// and operator arrays the neighbouring kernels reuse; NT = 0 lets factors that fit stay
+// resident across solves.
+template <int NT, typename T>
 __device__ __forceinline__ M4<T> ld4(const T* __restrict__ p, long c) {
   M4<T> r;
   if constexpr (sizeof(T) == 4) {
-    const float4 v = *reinterpret_cast<const float4*>(p + 4 * c);
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v* q = reinterpret_cast<const f4v*>(p + 4 * c);
+    f4v v;
+    if constexpr (NT) v = __builtin_nontemporal_load(q); else v = *q;
     r.m[0] = v.x; r.m[1] = v.y; r.m[2] = v.z; r.m[3] = v.w;
   } else {
-    const double2 v0 = *reinterpret_cast<const double2*>(p + 4 * c);
-    const double2 v1 = *reinterpret_cast<const double2*>(p + 4 * c + 2);
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v* q = reinterpret_cast<const d2v*>(p + 4 * c);
+    d2v v0, v1;
+    if constexpr (NT) {
+      v0 = __builtin_nontemporal_load(q);
+      v1 = __builtin_nontemporal_load(q + 1);
+    } else {
+      v0 = q[0];
+      v1 = q[1];
+    }
     r.m[0] = v0.x; r.m[1] = v0.y; r.m[2] = v1.x; r.m[3] = v1.y;
   }
   return r;
@@ -51,7 +66,10 @@ __device__ __forceinline__ V2<T> mtv(const M4<T>& q, V2<T> v) {  // q^T * v
 // row.  One thread per vertical column; each sweep works through windows of P levels whose
 // loads are all issued before the recurrence consumes them (the recurrence itself is
 // sequential, so memory-level parallelism has to come from the window).
-template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT>
+//
+// NT: cache policy of the factor loads and of the two vectors only this kernel reads
+// (p in the p-step, v in the s-step), see ld4.
+template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT, int NT = 1>
 __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j, int gxl,
                                           CT* stage, double (&acc)[1]) {
   const int nx = a.nx, nz = a.nz;
@@ -91,12 +109,25 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
       if constexpr (NP == 1) return V{(CT)p[c], (CT)0};
       else { V2<VT> t = ld2(p, c); return V{(CT)t.a, (CT)t.b}; }
     };
+    auto ldvo = [&](const VT* p, long c) -> V {  // a vector this kernel alone reads
+      if constexpr (NT == 0) return ldv(p, c);
+      else if constexpr (NP == 1) return V{(CT)__builtin_nontemporal_load(p + c), (CT)0};
+      else {
+        typedef VT v2v __attribute__((ext_vector_type(2)));
+        const v2v t = __builtin_nontemporal_load(reinterpret_cast<const v2v*>(p + 2 * c));
+        return V{(CT)t.x, (CT)t.y};
+      }
+    };
     auto stv = [&](VT* p, long c, V v) {
       if constexpr (NP == 1) p[c] = (VT)v.a; else st2(p, c, v.a, v.b);
     };
     auto ldm = [&](const CT* p, long c) -> M {
-      if constexpr (NP == 1) return M{{p[c], (CT)0, (CT)0, (CT)0}};
-      else return ld4(p, c);
+      if constexpr (NP == 1) {
+        if constexpr (NT) return M{{__builtin_nontemporal_load(p + c), (CT)0, (CT)0, (CT)0}};
+        else return M{{p[c], (CT)0, (CT)0, (CT)0}};
+      } else {
+        return ld4<NT>(p, c);
+      }
     };
     auto mul = [&](const M& m, V v) -> V {  // m * v
       if constexpr (NP == 1) return V{m.m[0] * v.a, (CT)0};
@@ -143,9 +174,9 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
           if constexpr (MODE == LINE_PLAIN) r_[q] = ldv(SRC, c);
           if constexpr (MODE == LINE_BICG_P) {
             r_[q] = ldv(SRC, c);
-            if (!first) { p_[q] = ldv((const VT*)PP, c); v_[q] = ldv(VV, c); }
+            if (!first) { p_[q] = ldvo((const VT*)PP, c); v_[q] = ldv(VV, c); }
           }
-          if constexpr (MODE == LINE_BICG_S) { r_[q] = ldv((const VT*)PP, c); v_[q] = ldv(VV, c); }
+          if constexpr (MODE == LINE_BICG_S) { r_[q] = ldv((const VT*)PP, c); v_[q] = ldvo(VV, c); }
           const long f = f0 + fs * k;
           if constexpr (TR == 0) { l_[q] = ldm(PL, f); t_[q] = ldm(PT, f); }
           else l_[q] = (k > 0) ? ldm(PW, f - fs) : M{{0, 0, 0, 0}};

@@ -13,7 +13,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/asam_sound.h"
+#include "asam_sound.h"
 #include "internal.h"
 
 using namespace asam;
@@ -133,6 +133,7 @@ extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning*
   t->line_tiled = c->g.tune.line_tiled;
   t->apply_band_rows = c->g.tune.apply_band_rows;
   t->apply_rows = c->g.tune.apply_rows;
+  t->line_nt = c->g.tune.line_nt;
   return 0;
 }
 
@@ -140,6 +141,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   if (!c || !t) return ASAM_E_INVALID;
   if (t->apply_variant < 0 || t->apply_variant > 3) return fail(c, ASAM_E_INVALID, "apply_variant");
   if (t->apply_rows != 2 && t->apply_rows != 4) return fail(c, ASAM_E_INVALID, "apply_rows");
+  if (t->line_nt < 0 || t->line_nt > 2) return fail(c, ASAM_E_INVALID, "line_nt");
   if (t->apply_band_rows < 0) return fail(c, ASAM_E_INVALID, "apply_band_rows");
   if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
   if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
@@ -159,6 +161,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.line_tiled = t->line_tiled ? 1 : 0;
   nt.apply_band_rows = t->apply_band_rows;
   nt.apply_rows = t->apply_rows;
+  nt.line_nt = t->line_nt;
   Geom tg = c->g;
   tg.tune = nt;
   if (apply_blocks(tg, 0) > kRedBlocks || apply_blocks(tg, 1) > kRedBlocks)
@@ -1465,6 +1468,16 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
       if (rc) { c->g.tune = t0; return rc; }
       if (m < best_ms) { best_ms = m; bt = c->g.tune; }
     }
+  // cache policy of the line solve's factor loads
+  const Tuning bn = bt;
+  for (int ntl : {0, 1}) {
+    c->g.tune = bn;
+    c->g.tune.line_nt = ntl;
+    double m = 0;
+    int rc = measure(&m);
+    if (rc) { c->g.tune = t0; return rc; }
+    if (m < best_ms) { best_ms = m; bt = c->g.tune; }
+  }
   // scalar tails in the producing kernels' last workgroups, or as finalize launches
   const Tuning bl = bt;
   for (int ft : {0, 1}) {

@@ -113,6 +113,10 @@ typedef struct {
   int line_tiled;    /* 1: line factors stored in 64-column tiles, levels contiguous */
   int apply_band_rows; /* variant 2: rows per band (0: ny/8, one band per XCD) */
   int apply_rows;    /* variant 3: rows of a column each thread walks together, 2 or 4 */
+  int line_nt;       /* cache policy of the line solve's factor loads (and of the vectors it
+                        reads once): 0 plain, 1 nontemporal, 2 (default) nontemporal when the
+                        factors of the solve's precision exceed 64 MiB, plain below, where
+                        they stay resident in the Infinity Cache.  Same bits either way */
 } asam_sound_tuning;
 
 /* ---- context ---------------------------------------------------------- */

@@ -257,6 +257,32 @@ def test_line_windows_agree(np1, transpose, oracle_mod, sound_mod):
             assert e < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
 
 
+@pytest.mark.parametrize("transpose", [False, True])
+@pytest.mark.parametrize("np1", [1, 2])
+def test_line_cache_policy_same_bits(np1, transpose, oracle_mod, sound_mod):
+    """The line solve's load cache policy (line_nt: plain, nontemporal, by footprint) changes
+    no bit of the FP64 or the mixed solve, forward or transposed; the FP64 solve matches the
+    oracle."""
+    p = W.config(1, np1=np1, scale=0.5)
+    xo, mo, _, _ = oracle_mod.Oracle(p).bicgstab(p.b, 300, 1e-8, transpose=transpose)
+    s = sound_mod.SoundSolver.from_problem(p)
+    solve = s.SolveSound_T if transpose else s.SolveSound
+    for prec in ("FP64", "MIXED"):
+        s.set_control(precision=prec)
+        ref = None
+        for nt in (0, 1, 2):
+            s.set_tuning(line_nt=nt)
+            x, mg, tg, info = solve(np.zeros(p.n), p.b, 300, 1e-8)
+            print(f"np1={np1} transpose={transpose} {prec} line_nt {nt}: vs oracle "
+                  f"{rel(x, xo):.2e}, status {info.status}")
+            assert info.status == 0
+            if ref is None:
+                ref = x.copy()
+                if prec == "FP64":
+                    assert rel(x, xo) < SOL_TOL and abs(abs(mg) - abs(mo)) <= 1
+            assert np.array_equal(x, ref)
+
+
 def test_zero_rhs_breakdown_semantics(oracle_mod, sound_mod):
     p = small(1)
     s = sound_mod.SoundSolver.from_problem(p)
