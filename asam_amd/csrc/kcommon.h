@@ -172,7 +172,8 @@ __device__ __forceinline__ void st2(T* p, long c, U a, U b) {
 
 // BiCGStab tail of one element (SpMatrix4CellPar_Mod.f90:322-331): x += alpha pHat +
 // omega rHat; r = s - omega t; accumulates r.r and rTilde.r.  FP32 vectors update in
-// FP32, the dots accumulate in FP64.
+// FP32, the dots accumulate in FP64.  x is touched by no other kernel of the iteration, so
+// its load and store are nontemporal (kept out of the caches the other vectors reuse).
 template <typename VT>
 __device__ __forceinline__ void bicg_upd_elem(long e, VT* __restrict__ x,
                                               const VT* __restrict__ ph,
@@ -181,20 +182,20 @@ __device__ __forceinline__ void bicg_upd_elem(long e, VT* __restrict__ x,
                                               const VT* __restrict__ rt, double alpha,
                                               double omega, double (&acc)[2]) {
   if constexpr (sizeof(VT) == 8) {
-    double xv = (double)x[e];
+    double xv = (double)__builtin_nontemporal_load(x + e);
     xv = xv + alpha * (double)ph[e];
     xv = xv + omega * (double)rh[e];
     double rv = (double)r[e] - omega * (double)t[e];
-    x[e] = (VT)xv;
+    __builtin_nontemporal_store((VT)xv, x + e);
     r[e] = (VT)rv;
     acc[0] += rv * rv;
     acc[1] += (double)rt[e] * rv;
   } else {
-    float xf = (float)x[e];
+    float xf = (float)__builtin_nontemporal_load(x + e);
     xf = xf + (float)alpha * (float)ph[e];
     xf = xf + (float)omega * (float)rh[e];
     float rf = (float)r[e] - (float)omega * (float)t[e];
-    x[e] = (VT)xf;
+    __builtin_nontemporal_store((VT)xf, x + e);
     r[e] = (VT)rf;
     acc[0] += (double)rf * (double)rf;
     acc[1] += (double)rt[e] * (double)rf;

@@ -67,8 +67,9 @@ __device__ __forceinline__ V2<T> mtv(const M4<T>& q, V2<T> v) {  // q^T * v
 // loads are all issued before the recurrence consumes them (the recurrence itself is
 // sequential, so memory-level parallelism has to come from the window).
 //
-// NT: cache policy of the factor loads and of the two vectors only this kernel reads
-// (p in the p-step, v in the s-step), see ld4.
+// NT: cache policy of the factor loads, of the two vectors only this kernel reads (p in
+// the p-step, v in the s-step) and of the p-step's store of p (read again only by the next
+// iteration's p-step), see ld4.
 template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT, int NT = 1>
 __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j, int gxl,
                                           CT* stage, double (&acc)[1]) {
@@ -116,6 +117,14 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
         typedef VT v2v __attribute__((ext_vector_type(2)));
         const v2v t = __builtin_nontemporal_load(reinterpret_cast<const v2v*>(p + 2 * c));
         return V{(CT)t.x, (CT)t.y};
+      }
+    };
+    auto stvn = [&](VT* p, long c, V v) {
+      if constexpr (NP == 1) __builtin_nontemporal_store((VT)v.a, p + c);
+      else {
+        typedef VT v2v __attribute__((ext_vector_type(2)));
+        v2v t; t.x = (VT)v.a; t.y = (VT)v.b;
+        __builtin_nontemporal_store(t, reinterpret_cast<v2v*>(p + 2 * c));
       }
     };
     auto stv = [&](VT* p, long c, V v) {
@@ -193,7 +202,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
             in = first ? r_[q]
                        : V{r_[q].a + beta * (p_[q].a - omega * v_[q].a),
                            r_[q].b + beta * (p_[q].b - omega * v_[q].b)};
-            stv(PP, c, in);
+            if constexpr (NT) stvn(PP, c, in); else stv(PP, c, in);
           } else {
             in = V{r_[q].a - alpha * v_[q].a, r_[q].b - alpha * v_[q].b};
             stv(PP, c, in);
