@@ -172,8 +172,9 @@ __device__ __forceinline__ void st2(T* p, long c, U a, U b) {
 
 // BiCGStab tail of one element (SpMatrix4CellPar_Mod.f90:322-331): x += alpha pHat +
 // omega rHat; r = s - omega t; accumulates r.r and rTilde.r.  FP32 vectors update in
-// FP32, the dots accumulate in FP64.  x is touched by no other kernel of the iteration, so
-// its load and store are nontemporal (kept out of the caches the other vectors reuse).
+// FP32, the dots accumulate in FP64.  x is touched by no other kernel of the iteration, and
+// this is the iteration's last read of pHat, rHat and t: those loads and x's store are
+// nontemporal (kept out of the caches the other vectors reuse).
 template <typename VT>
 __device__ __forceinline__ void bicg_upd_elem(long e, VT* __restrict__ x,
                                               const VT* __restrict__ ph,
@@ -183,18 +184,18 @@ __device__ __forceinline__ void bicg_upd_elem(long e, VT* __restrict__ x,
                                               double omega, double (&acc)[2]) {
   if constexpr (sizeof(VT) == 8) {
     double xv = (double)__builtin_nontemporal_load(x + e);
-    xv = xv + alpha * (double)ph[e];
-    xv = xv + omega * (double)rh[e];
-    double rv = (double)r[e] - omega * (double)t[e];
+    xv = xv + alpha * (double)__builtin_nontemporal_load(ph + e);
+    xv = xv + omega * (double)__builtin_nontemporal_load(rh + e);
+    double rv = (double)r[e] - omega * (double)__builtin_nontemporal_load(t + e);
     __builtin_nontemporal_store((VT)xv, x + e);
     r[e] = (VT)rv;
     acc[0] += rv * rv;
     acc[1] += (double)rt[e] * rv;
   } else {
     float xf = (float)__builtin_nontemporal_load(x + e);
-    xf = xf + (float)alpha * (float)ph[e];
-    xf = xf + (float)omega * (float)rh[e];
-    float rf = (float)r[e] - (float)omega * (float)t[e];
+    xf = xf + (float)alpha * (float)__builtin_nontemporal_load(ph + e);
+    xf = xf + (float)omega * (float)__builtin_nontemporal_load(rh + e);
+    float rf = (float)r[e] - (float)omega * (float)__builtin_nontemporal_load(t + e);
     __builtin_nontemporal_store((VT)xf, x + e);
     r[e] = (VT)rf;
     acc[0] += (double)rf * (double)rf;
