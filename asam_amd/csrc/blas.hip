@@ -50,8 +50,16 @@ __global__ __launch_bounds__(256) void k_bicg_update(long n, VT* __restrict__ x,
                                                      const VT* __restrict__ rh,
                                                      VT* __restrict__ r, const VT* __restrict__ t,
                                                      const VT* __restrict__ rt, const KState* st,
-                                                     double* partial, KTail tail) {
-  if (st->done) return;
+                                                     double* partial, KTail tail, KTail fold) {
+  if (st->done) return;  // the state before this launch: the same in every workgroup
+  // a folded tail (the half-step test and omega): the scalars come from this workgroup's
+  // copy of the state
+  __shared__ unsigned long long sk[kTailWords];
+  if (fold.kind >= 0) {
+    fold_state_256(fold, sk, threadIdx.x);
+    fold_write(fold, sk, threadIdx.x, 256);
+    st = reinterpret_cast<const KState*>(sk);
+  }
   const double alpha = st->alpha, omega = st->omega;
   const bool half = st->half != 0;
   double acc[2] = {0.0, 0.0};
@@ -69,16 +77,16 @@ __global__ __launch_bounds__(256) void k_bicg_update(long n, VT* __restrict__ x,
 
 void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,
                         void* r, const void* t, const void* rt, const KState* st,
-                        double* partial, const KTail& tail) {
+                        double* partial, const KTail& tail, const KTail& fold) {
   int nb = blas_blocks(n);
   if (vt == 0)
     hipLaunchKernelGGL(k_bicg_update<double>, dim3(nb), dim3(256), 0, s, n, (double*)x,
                        (const double*)ph, (const double*)rh, (double*)r, (const double*)t,
-                       (const double*)rt, st, partial, tail);
+                       (const double*)rt, st, partial, tail, fold);
   else
     hipLaunchKernelGGL(k_bicg_update<float>, dim3(nb), dim3(256), 0, s, n, (float*)x,
                        (const float*)ph, (const float*)rh, (float*)r, (const float*)t,
-                       (const float*)rt, st, partial, tail);
+                       (const float*)rt, st, partial, tail, fold);
 }
 
 // ------------------------------------------------------------- (F)GMRES MGS

@@ -48,7 +48,9 @@ struct Tuning {
   int apply_band_rows = 0;  // band-ordered apply: rows per band (0: ny/8)
   int apply_rows = 2;     // variant 3: rows per thread (2 or 4)
   int fused_tail = 0;     // 1: BiCGStab and refinement tails run by the producing kernel's
-                          // last workgroup (0: a separate finalize launch each)
+                          // last workgroup; 2: the BiCGStab iteration's tails folded into
+                          // the kernels that consume their scalars (0: a separate finalize
+                          // launch each)
   int line_nt = 2;        // line solve's factor loads (and the vectors it reads once):
                           // 0 plain, 1 nontemporal, 2 nontemporal when the factors of the
                           // solve's precision exceed kLineNtBytes
@@ -106,17 +108,27 @@ struct KState {
   double cs[kMaxM], sn[kMaxM], bhat[kMaxM + 1], y[kMaxM], vscale[kMaxM + 1];
 };
 
-// Scalar tail of a reduction run by the last workgroup of the producing kernel in place
-// of a finalize launch (kind < 0: none).  The kinds are those whose step reads and
-// writes only the state's scalars ahead of H.  counter counts the arriving workgroups;
-// it is zero between launches (the last workgroup resets it).
+// Scalar tail of a reduction run inside a kernel in place of a finalize launch (kind < 0:
+// none): by the last workgroup of the kernel that produced the partial sums (fused tail),
+// or by every workgroup of the kernel that needs the scalars (folded tail).  The kinds are
+// those whose step reads and writes only the state's scalars ahead of H.  counter counts
+// the arriving workgroups; it is zero between launches (the last workgroup resets it).
 struct KTail {
   int kind = -1;
   int nq = 0, nblk = 0, nblk_hi = 0;  // slots, partials of slots 0-1 and of slots 2-3
+  int qmask = 0xF;                    // slots summed (the others enter the step as 0)
   const double* partial = nullptr;
   KState* st = nullptr;
   unsigned int* counter = nullptr;
+  // folded tails: the state is read from src (null: st) and also written to copy
+  const KState* src = nullptr;
+  KState* copy = nullptr;
 };
+
+// Partial-sum buffer: two regions of 4 slots of kRedBlocks.  With folded tails the
+// BiCGStab update writes its sums to the second region, because the update's own
+// workgroups still read the first while they fold the previous tail.
+constexpr long kPartialRegion = 4L * kRedBlocks;
 
 // ------------------------------------------------------------------ set-up
 void launch_weights(hipStream_t s, const Geom& g, const double* rho, const double* rth,
@@ -183,6 +195,7 @@ struct LineReq {
   const int* done = nullptr;
   int use_lds = 1;
   int qslot = 0;  // partial-sum quantity slot of the LINE_BICG_S norm
+  KTail fold;     // tail folded into the launch: every workgroup runs the scalar step first
 };
 int line_blocks(const Geom& g);
 void launch_line(hipStream_t s, const Geom& g, const Coef<double>& c64, const Coef<float>& c32,
@@ -194,9 +207,11 @@ int blas_blocks(long n);
 void launch_any_nonzero(hipStream_t s, long n, const double* x, int* xnz, int* xz);
 // BiCGStab tail: x += alpha pHat + omega rHat; r -= omega t; q0 = r.r, q1 = rt.r.
 // With st->half set: x += alpha pHat only.
+// fold: the half-step/omega tail of the preceding stencil apply, folded into the launch.
 void launch_bicg_update(hipStream_t s, long n, int vt, void* x, const void* ph, const void* rh,
                         void* r, const void* t, const void* rt, const KState* st,
-                        double* partial, const KTail& tail = KTail());
+                        double* partial, const KTail& tail = KTail(),
+                        const KTail& fold = KTail());
 // (F)GMRES modified Gram-Schmidt step: w -= H[k-1][i]*vscale[k-1]*v_{k-1} (k>0); then
 // q0 = w.v_k (k<=i) or w.w (k==i+1), into partial slot 0 of `partial`.  hsrc: null (H[k-1][i]
 // is in the state), or the previous step's partial products, whose sum is H[k-1][i]

@@ -64,6 +64,52 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NQ], double* part
 // Scalar words of the state ahead of the (F)GMRES arrays: all that the fused tails touch.
 constexpr int kTailWords = (int)(offsetof(KState, H) / 8);
 
+// The summed quantities S[q] of the partial sums t describes, in k_finalize's order, by a
+// workgroup of 256 threads (see tail_arrive); slots outside t.qmask are 0.
+__device__ __forceinline__ void tail_sums_256(const KTail& t, int tid, double (&S)[4]) {
+  __shared__ double sh[4][16];
+  __shared__ double vs[1024];
+  const int lane = tid & 63, wid = tid >> 6;
+  for (int q = 0; q < 4; ++q) {
+    if (q >= t.nq || !((t.qmask >> q) & 1)) continue;  // uniform
+    double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nb = q < 2 ? t.nblk : t.nblk_hi;
+    const double* pq = t.partial + (long)q * kRedBlocks;
+    // full rounds of 1024 first (no lane condition, so their loads are in flight
+    // together), then the partial last round
+    const int full = nb >> 10;
+#pragma unroll 4
+    for (int r = 0; r < full; ++r) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s4[c] += pq[(r << 10) + tid + (c << 8)];
+    }
+    if ((full << 10) < nb) {
+      // clamped unconditional loads, the value selected after (adding +0.0 keeps the bits)
+      double x[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x[c] = pq[min((full << 10) + tid + (c << 8), nb - 1)];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s4[c] += ((full << 10) + tid + (c << 8) < nb) ? x[c] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) vs[tid + (c << 8)] = s4[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int g = wid + (c << 2);
+      const double v = wave_sum(vs[(g << 6) + lane]);
+      if (lane == 0) sh[q][g] = v;
+    }
+    __syncthreads();
+  }
+  for (int q = 0; q < 4; ++q) {
+    double s = 0.0;
+    if (q < t.nq && ((t.qmask >> q) & 1))
+      for (int w = 0; w < 16; ++w) s += sh[q][w];
+    S[q] = s;
+  }
+}
+
 // Fused scalar tail, in place of a k_finalize launch.  Every thread of every workgroup
 // of 256 threads calls it right after block_reduce_store.
 //
@@ -84,8 +130,6 @@ constexpr int kTailWords = (int)(offsetof(KState, H) / 8);
 __device__ __forceinline__ void tail_arrive(const KTail& t, int tid) {
   __shared__ int s_last;
   __shared__ unsigned long long sk[kTailWords];
-  __shared__ double sh[4][16];
-  __shared__ double vs[1024];
   if (tid == 0) {
     // completion of this wave's stores, the partial sums included; also a compiler
     // barrier, so the atomic add below is not issued before them
@@ -101,52 +145,85 @@ __device__ __forceinline__ void tail_arrive(const KTail& t, int tid) {
   if (!s_last) return;
   for (int w = tid; w < kTailWords; w += 256)
     sk[w] = reinterpret_cast<const unsigned long long*>(t.st)[w];
-  const int lane = tid & 63, wid = tid >> 6;
-  for (int q = 0; q < 4; ++q) {
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-    if (q < t.nq) {
-      const int nb = q < 2 ? t.nblk : t.nblk_hi;
-      const double* pq = t.partial + (long)q * kRedBlocks;
-      // full rounds of 1024 first (no lane condition, so their loads are in flight
-      // together), then the partial last round
-      const int full = nb >> 10;
-#pragma unroll 4
-      for (int r = 0; r < full; ++r) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) s4[c] += pq[(r << 10) + tid + (c << 8)];
-      }
-      if ((full << 10) < nb) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int b = (full << 10) + tid + (c << 8);
-          if (b < nb) s4[c] += pq[b];
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) vs[tid + (c << 8)] = s4[c];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int g = wid + (c << 2);
-      const double v = wave_sum(vs[(g << 6) + lane]);
-      if (lane == 0) sh[q][g] = v;
-    }
-    __syncthreads();
-  }
+  double S[4];
+  tail_sums_256(t, tid, S);
   if (tid == 0) {
-    double S[4];
-    for (int q = 0; q < 4; ++q) {
-      double s = 0.0;
-      for (int w = 0; w < 16; ++w) s += sh[q][w];
-      S[q] = q < t.nq ? s : 0.0;
-    }
     fin_logic(t.kind, S, reinterpret_cast<KState*>(sk), 0, 0, t.nq, nullptr);
     __hip_atomic_store(t.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
   for (int w = tid; w < kTailWords; w += 256)
     reinterpret_cast<unsigned long long*>(t.st)[w] = sk[w];
+}
+
+// Folded tail, in place of a finalize launch between the kernel that produced the partial
+// sums and the one that needs the scalars: every workgroup of the consumer loads the
+// state's scalars into LDS (sk), adds the partial sums in k_finalize's order and runs the
+// scalar step on its copy, then reads its scalars from the copy.  The partial sums come
+// from an earlier launch, so no fence is needed.  The workgroups hold identical copies and
+// the first one writes its copy back (fold_write).  No counter: an arrival count on one
+// address serialises a thousand workgroups' atomics.  That is safe when the step writes
+// none of the fields it reads (alpha; the half-step test and omega); a step that does
+// (the exit test and beta) reads src, a copy of the state that no workgroup of the launch
+// writes.
+//
+// One wavefront of 64 threads: virtual thread v = 64 w + lane of k_finalize's 1024, the
+// 16 wave totals added in order.
+__device__ __forceinline__ void fold_state_w64(const KTail& f, unsigned long long* sk) {
+  const int lane = threadIdx.x & 63;
+  const KState* src = f.src ? f.src : f.st;
+  for (int w = lane; w < kTailWords; w += 64)
+    sk[w] = reinterpret_cast<const unsigned long long*>(src)[w];
+  double S[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    S[q] = 0.0;
+    if (q >= f.nq || !((f.qmask >> q) & 1)) continue;  // uniform
+    const int nb = q < 2 ? f.nblk : f.nblk_hi;
+    const double* pq = f.partial + (long)q * kRedBlocks;
+    // every load of a round unconditional (a clamped index, the value selected after), so
+    // the 16 are in flight together; adding +0.0 for a missing partial keeps the bits
+    double vw[16];
+#pragma unroll
+    for (int w = 0; w < 16; ++w) vw[w] = 0.0;
+    const int rounds = (nb + 1023) >> 10;
+    for (int r = 0; r < rounds; ++r) {
+      double x[16];
+#pragma unroll
+      for (int w = 0; w < 16; ++w) x[w] = pq[min((r << 10) + (w << 6) + lane, nb - 1)];
+#pragma unroll
+      for (int w = 0; w < 16; ++w) vw[w] += ((r << 10) + (w << 6) + lane < nb) ? x[w] : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) s += wave_sum(vw[w]);
+    S[q] = s;
+  }
+  __syncthreads();
+  if (lane == 0) fin_logic(f.kind, S, reinterpret_cast<KState*>(sk), 0, 0, f.nq, nullptr);
+  __syncthreads();
+}
+
+// The same for a workgroup of 256 threads.
+__device__ __forceinline__ void fold_state_256(const KTail& f, unsigned long long* sk, int tid) {
+  const KState* src = f.src ? f.src : f.st;
+  for (int w = tid; w < kTailWords; w += 256)
+    sk[w] = reinterpret_cast<const unsigned long long*>(src)[w];
+  double S[4];
+  tail_sums_256(f, tid, S);
+  __syncthreads();
+  if (tid == 0) fin_logic(f.kind, S, reinterpret_cast<KState*>(sk), 0, 0, f.nq, nullptr);
+  __syncthreads();
+}
+
+// The first workgroup of the launch writes its state copy to st (and to copy).
+__device__ __forceinline__ void fold_write(const KTail& f, const unsigned long long* sk,
+                                           int tid, int nthreads) {
+  if (blockIdx.x != 0 || blockIdx.y != 0 || blockIdx.z != 0) return;
+  for (int w = tid; w < kTailWords; w += nthreads) {
+    reinterpret_cast<unsigned long long*>(f.st)[w] = sk[w];
+    if (f.copy) reinterpret_cast<unsigned long long*>(f.copy)[w] = sk[w];
+  }
 }
 
 template <typename T> struct V2 { T a, b; };

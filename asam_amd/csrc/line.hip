@@ -48,14 +48,32 @@ int lds_levels(int nz, int np, size_t word, size_t budget) {
 // One 64-column tile per workgroup: grid (gxl, ny).
 template <int NP, int TR, int MODE, int LDS, int P, int NT, typename CT, typename VT>
 __global__ __launch_bounds__(64) void k_line(KLine<CT, VT> a) {
-  if (a.done && *a.done) return;
+  if (a.done && *a.done) return;  // the state before this launch: the same in every workgroup
   extern __shared__ unsigned char lds_raw[];
   CT* stage = reinterpret_cast<CT*>(lds_raw);
   const int blk = blockIdx.x + gridDim.x * blockIdx.y;
   double acc[1] = {0.0};
-  line_tile<NP, TR, MODE, LDS, P, CT, VT, NT>(a, blockIdx.x, blockIdx.y, gridDim.x, stage, acc);
-  if constexpr (MODE == LINE_BICG_S) {
-    block_reduce_store<1>(acc, a.partial + (long)a.qslot * kRedBlocks, blk, threadIdx.x, 64);
+  if constexpr (MODE == LINE_PLAIN) {
+    line_tile<NP, TR, MODE, LDS, P, CT, VT, NT>(a, blockIdx.x, blockIdx.y, gridDim.x, stage, acc);
+  } else {
+    // a folded tail (the previous iteration's exit test and beta before the p-step, alpha
+    // before the s-step): the scalars come from this workgroup's copy of the state
+    __shared__ unsigned long long sk[kTailWords];
+    const bool fold = a.fold.kind >= 0;
+    bool run = true;
+    KLine<CT, VT> b = a;
+    if (fold) {
+      fold_state_w64(a.fold, sk);
+      fold_write(a.fold, sk, threadIdx.x, 64);
+      const KState* s = reinterpret_cast<const KState*>(sk);
+      b.st = s;
+      run = s->done == 0;
+    }
+    if (run) line_tile<NP, TR, MODE, LDS, P, CT, VT, NT>(b, blockIdx.x, blockIdx.y, gridDim.x,
+                                                         stage, acc);
+    if constexpr (MODE == LINE_BICG_S) {
+      block_reduce_store<1>(acc, a.partial + (long)a.qslot * kRedBlocks, blk, threadIdx.x, 64);
+    }
   }
 }
 
@@ -100,6 +118,7 @@ static void line_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const LineRe
   a.src = (const VT*)r.src; a.p = (VT*)r.p; a.v = (const VT*)r.v; a.out = (VT*)r.out;
   a.scale = r.scale; a.st = r.st; a.partial = r.partial; a.done = r.done;
   a.qslot = r.qslot;
+  a.fold = r.fold;
   a.kl = r.use_lds ? lds_levels(g.nz, NP, sizeof(CT), lds_budget_for(g.tune.line_lds)) : 0;
   const size_t lds = (size_t)a.kl * 64 * NP * sizeof(CT);
   const bool use_lds = a.kl > 0;

@@ -21,6 +21,7 @@ struct KLine {
   const int* done;
   int qslot;
   int kl;  // LDS staging: levels k < kl are staged in LDS, the rest in the output vector
+  KTail fold;  // scalar tail folded into the launch (k_line only; kind < 0: none)
 };
 
 template <typename T> struct M4 { T m[4]; };

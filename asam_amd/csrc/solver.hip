@@ -152,7 +152,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.line_window = t->line_window;
   if (t->line_lds < 0 || t->line_lds > 160) return fail(c, ASAM_E_INVALID, "line_lds");
   nt.line_lds = t->line_lds;
-  if (t->fused_tail != 0 && t->fused_tail != 1) return fail(c, ASAM_E_INVALID, "fused_tail");
+  if (t->fused_tail < 0 || t->fused_tail > 2) return fail(c, ASAM_E_INVALID, "fused_tail");
   nt.fused_tail = t->fused_tail;
   if (t->apply_bx != 32 && t->apply_bx != 64 && t->apply_bx != 128 && t->apply_bx != 256)
     return fail(c, ASAM_E_INVALID, "apply_bx");
@@ -239,9 +239,9 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
   long nf = (long)(grid->nx + 1) * grid->ny * grid->nz + (long)grid->nx * (grid->ny + 1) * grid->nz +
             (long)grid->nx * grid->ny * (grid->nz + 1);
   rc |= dalloc(c, &c->thfaces, nf);
-  rc |= dalloc(c, &c->partial, 4L * kRedBlocks);
+  rc |= dalloc(c, &c->partial, 2 * kPartialRegion);
   rc |= dalloc(c, &c->tail_ctr, 64);
-  rc |= dalloc(c, &c->st, 1);
+  rc |= dalloc(c, &c->st, 2);  // the state, and a copy the folded tails read (st + 1)
   rc |= dalloc(c, &c->dead, N);
   const long n = c->n;
   double** v64[] = {&c->r, &c->rt, &c->p, &c->ph, &c->v, &c->rh, &c->t, &c->xd, &c->bd, &c->xalt};
@@ -426,7 +426,7 @@ struct Enq {
   // The tails whose step touches only the state's scalars run inside the producing
   // kernel when the tuning asks for it; the others keep their finalize launch.
   bool fused(int kind) const {
-    return c->g.tune.fused_tail &&
+    return c->g.tune.fused_tail == 1 &&
            (kind == FIN_BICG_INIT || kind == FIN_BICG_ALPHA || kind == FIN_BICG_HALF_OMEGA ||
             kind == FIN_BICG_END || kind == FIN_MIXED_OUTER);
   }
@@ -435,6 +435,23 @@ struct Enq {
     t.kind = kind; t.nq = nq; t.nblk = nblk; t.nblk_hi = nblk_hi > 0 ? nblk_hi : nblk;
     t.partial = c->partial; t.st = c->st; t.counter = c->tail_ctr;
     return t;
+  }
+  // BiCGStab tails folded into the kernels that consume their scalars (tuning
+  // fused_tail = 2): alpha into the s-step, the half-step test and omega into the update,
+  // the exit test and beta into the next iteration's p-step.  The update then writes its
+  // sums to the second partial region, and its tail stays pending until that p-step, or
+  // until flush() launches it at the end of a batch of iterations.
+  bool folding() const { return c->g.tune.fused_tail == 2; }
+  bool end_pending = false;
+  KTail folded(int kind, int nq, int nblk, int nblk_hi, const double* partial, int qmask) const {
+    KTail t = tail(kind, nq, nblk, nblk_hi);
+    t.partial = partial; t.qmask = qmask; t.counter = nullptr;
+    return t;
+  }
+  void flush() {
+    if (!end_pending) return;
+    end_pending = false;
+    fin(FIN_BICG_END, 2, blas_blocks(c->n), 0, 0, 0, 0, c->partial + kPartialRegion);
   }
 
   hipEvent_t ev() {
@@ -503,11 +520,11 @@ struct Enq {
     if (fkind >= 0 && !fuse) fin(fkind, 1, apply_blocks(c->g, 0));
   }
   void line(int prec, int mode, const void* src, void* p, const void* v, void* out,
-            const double* scale, const int* done, int qslot = 0) {
+            const double* scale, const int* done, int qslot = 0, const KTail& fold = KTail()) {
     LineReq r;
     r.tr = tr; r.prec = prec; r.vt = prec; r.mode = mode; r.src = src; r.p = p; r.v = v;
     r.out = out; r.scale = scale; r.st = c->st; r.partial = c->partial; r.done = done;
-    r.qslot = qslot;
+    r.qslot = qslot; r.fold = fold;
     timed(2 + prec, line_bytes(c->g, r, iter == 1),
           [&] { launch_line(s, c->g, c->c64, c->c32, r); });
     c->counts[1]++;
@@ -524,11 +541,22 @@ struct Enq {
     const double w = vt ? 4 : 8, nv = (double)c->n;
     const bool fuse = fused(FIN_BICG_END);
     const KTail tl = fuse ? tail(FIN_BICG_END, 2, blas_blocks(c->n), 0) : KTail();
+    const bool fold = folding();
+    // folded: the half-step test and omega of the preceding apply (slots 0-1) and s-step
+    // (slot 2) are run by every workgroup; the sums go to the second region
+    KTail fo;
+    if (fold) {
+      fo = folded(FIN_BICG_HALF_OMEGA, 3, apply_blocks(c->g, vt), line_blocks(c->g), c->partial,
+                  0x7);
+      fo.copy = c->st + 1;  // the exit test folded into the next p-step reads this copy
+    }
+    double* part = fold ? c->partial + kPartialRegion : c->partial;
     timed(4 + vt, 8 * nv * w,
-          [&] { launch_bicg_update(s, c->n, vt, x, ph, rh, r, t, rt, c->st, c->partial, tl); },
+          [&] { launch_bicg_update(s, c->n, vt, x, ph, rh, r, t, rt, c->st, part, tl, fo); },
           2, 3 * nv * w);
     c->counts[2]++;
-    if (!fuse) fin(FIN_BICG_END, 2, blas_blocks(c->n));
+    if (fold) end_pending = true;
+    else if (!fuse) fin(FIN_BICG_END, 2, blas_blocks(c->n));
   }
   // classical Gram-Schmidt: nq projections of w on basis vectors k0.., then the removal
   // of all i + 1 projections with |w|^2
@@ -606,8 +634,10 @@ int upload_state(asam_sound_ctx* c) {
 }
 
 // One BiCGStab iteration (SpMatrix4CellPar_Mod.f90:280-337) as 5 fused kernels; the
-// scalar tails (alpha; half-step test and omega; exit tests and beta) run in the last
-// workgroup of the kernel before them, or as finalize launches (fused_tail = 0).
+// scalar tails (alpha; half-step test and omega; exit tests and beta) run as finalize
+// launches (fused_tail = 0), in the last workgroup of the kernel before them (1), or in
+// every workgroup of the kernel after them (2; the exit tests and beta of the last
+// iteration of a batch then run at the batch's flush()).
 // prec/vt: 0 FP64, 1 FP32 vectors and coefficients (dots always FP64).
 template <typename VT>
 void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v, VT* rh, VT* t) {
@@ -615,16 +645,27 @@ void bicg_iteration(Enq& e, int prec, VT* x, VT* r, VT* rt, VT* p, VT* ph, VT* v
   KState* st = c->st;
   const Geom& g = c->g;
   const int vt = prec;
-  // p = r + beta (p - omega v) (p = r at iter 1); pHat = PSolve(p)
-  e.line(prec, LINE_BICG_P, r, p, v, ph, nullptr, &st->done);
-  // v = MatVec(pHat); alpha = rho / (rTilde.v)
+  const bool fold = e.folding();
+  // p = r + beta (p - omega v) (p = r at iter 1); pHat = PSolve(p).  Folded: the previous
+  // iteration's exit test and beta first (the update's sums, second region)
+  KTail fend;
+  if (fold && e.end_pending) {
+    fend = e.folded(FIN_BICG_END, 2, blas_blocks(c->n), 0, c->partial + kPartialRegion, 0x3);
+    fend.src = c->st + 1;  // the update's copy: the step rewrites rho, iter and done
+    e.end_pending = false;
+  }
+  e.line(prec, LINE_BICG_P, r, p, v, ph, nullptr, &st->done, 0, fend);
+  // v = MatVec(pHat); alpha = rho / (rTilde.v) (folded: in the s-step)
   e.apply(prec, vt, vt, vt, ph, v, nullptr, nullptr, rt, DOT_YU, &st->done, nullptr, nullptr, 0,
-          FIN_BICG_ALPHA, 2);
+          fold ? -1 : FIN_BICG_ALPHA, 2);
   // s = r - alpha v (into r); |s| (partial slot 2); rHat = PSolve(s)
-  e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done, 2);
+  e.line(prec, LINE_BICG_S, nullptr, r, v, rh, nullptr, &st->done, 2,
+         fold ? e.folded(FIN_BICG_ALPHA, 2, apply_blocks(g, prec), 0, c->partial, 0x2)
+              : KTail());
   // t = MatVec(rHat); t.t, t.s; then the half-step exit test on |s| or omega = (t.s)/(t.t)
+  // (folded: in the update)
   e.apply(prec, vt, vt, vt, rh, t, nullptr, nullptr, r, DOT_YY | DOT_YU, &st->done, nullptr,
-          nullptr, 0, FIN_BICG_HALF_OMEGA, 3, line_blocks(g));
+          nullptr, 0, fold ? -1 : FIN_BICG_HALF_OMEGA, 3, line_blocks(g));
   // x += alpha pHat + omega rHat; r = s - omega t (half exit: x += alpha pHat);
   // |r|, rTilde.r; exit tests, beta
   e.bicg_update(vt, x, ph, rh, r, t, rt);
@@ -647,6 +688,7 @@ int bicg_loop(Enq& e, int prec, int maxiter, VT* x, VT* r, VT* rt, VT* p, VT* ph
       bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
     }
     issued += q;
+    e.flush();  // the last iteration's exit test, when folded tails left it pending
     int rc = sync_state(c);
     if (rc) return rc;
     if (c->prof_on) {
@@ -861,6 +903,7 @@ static int solve_mixed(asam_sound_ctx* c, int tr, double* x, const double* b, do
                               c->t32);
       }
       e.iter = 0;
+      e.flush();
       outer_step(k);
     }
     k = ns;
@@ -1383,6 +1426,7 @@ static int tune_iters(asam_sound_ctx* c, int prec, int iters, double* ms) {
   Enq e(c, 0);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   for (int k = 0; k < iters; ++k) bicg_iteration<VT>(e, prec, x, r, rt, p, ph, v, rh, t);
+  e.flush();
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   HIPCHK(c, hipEventSynchronize(c->ev1));
   HIPCHK(c, hipGetLastError());
@@ -1478,9 +1522,10 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
     if (rc) { c->g.tune = t0; return rc; }
     if (m < best_ms) { best_ms = m; bt = c->g.tune; }
   }
-  // scalar tails in the producing kernels' last workgroups, or as finalize launches
+  // scalar tails as finalize launches, in the producing kernels' last workgroups, or
+  // folded into the consuming kernels
   const Tuning bl = bt;
-  for (int ft : {0, 1}) {
+  for (int ft : {0, 1, 2}) {
     c->g.tune = bl;
     c->g.tune.fused_tail = ft;
     double m = 0;

@@ -104,9 +104,12 @@ typedef struct {
                         in the output vector */
   int fused_tail;    /* 1: the scalar steps of BiCGStab and of the mixed refinement run
                         in the last workgroup of the kernel that produced their sums;
-                        0 (default): a separate finalize launch each.  Same bits either
-                        way; the autotuner times both.  (F)GMRES tails are always
-                        separate launches */
+                        2: the three scalar steps of a BiCGStab iteration run in every
+                        workgroup of the kernel that needs their scalars (alpha in the
+                        s-step line solve, omega in the update, the exit test and beta in
+                        the next p-step line solve); 0 (default): a separate finalize
+                        launch each.  Same bits in all three; the autotuner times them.
+                        (F)GMRES tails are always separate launches */
   int apply_xcd;     /* 1: contiguous tile ranges per XCD, 0: round-robin order */
   int apply_bx;      /* apply workgroup width along x: 32, 64, 128 or 256 (256 threads; at 32
                         a wavefront covers two stacked row blocks of variant 3) */
