@@ -295,15 +295,18 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
   }
 }
 
-// Row-blocked z-march variant: R rows per thread (march_rows), tiles of an XCD contiguous.
-template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
+// Row-blocked z-march variant: R rows per thread (march_rows; DEEP: march_rows_deep, loads
+// two levels ahead), tiles of an XCD contiguous.
+template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU,
+          int DEEP = 0>
 __global__ __launch_bounds__(256) void k_apply_rows(KApply<CT, VX, VO, VU> a) {
   if (a.done && *a.done) return;
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
   const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
   double acc[2] = {0.0, 0.0};
-  march_rows<NP, TR, FL, R, CT, VX, VO, VU>(a, L, acc);
+  if constexpr (DEEP) march_rows_deep<TR, FL, R, CT, VX, VO, VU>(a, L, acc);
+  else march_rows<NP, TR, FL, R, CT, VX, VO, VU>(a, L, acc);
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
     if (a.tail.kind >= 0) tail_arrive(a.tail, tid);
@@ -332,6 +335,15 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
     // applies of a mixed solve share the FP32 inner solve's tuning.
     bool four = false;
     if constexpr (sizeof(CT) == 4) four = g.tune.apply_rows == 4;
+    // loads two levels ahead: FP32 coefficients, 2x2 blocks and two rows (the FP32 inner
+    // solve's applies); FP64 kernels and four rows would spill the extra registers
+    if constexpr (sizeof(CT) == 4 && NP == 2 && (FL & AF_X0) == 0) {
+      if (g.tune.apply_depth == 2 && !four) {
+        hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 2, CT, VX, VO, VU, 1>), dim3(sh.nblocks),
+                           dim3(a.bxw, a.byw), 0, s, a);
+        return;
+      }
+    }
     if (four) {
       if constexpr (sizeof(CT) == 4)
         hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 4, CT, VX, VO, VU>), dim3(sh.nblocks),

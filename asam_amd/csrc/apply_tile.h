@@ -551,4 +551,198 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
   }
 }
 
+// march_rows for 2x2 blocks with the loads two levels deep: iteration k issues the cell
+// data (x, DUT, volume; west/east neighbours; halo rows) of level k + 2 and the face
+// coefficients and dot operands of level k + 1, then computes level k from registers
+// filled one iteration earlier.  Two iterations of loads are in flight per thread instead
+// of one.  The arithmetic per cell is march_rows' (the same bits).  Levels past the top
+// are read at the top level: in bounds, and the top z-face coefficients are zero.
+template <int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
+__device__ __forceinline__ void march_rows_deep(const KApply<CT, VX, VO, VU>& a, int L,
+                                                double (&acc)[2]) {
+  static_assert((FL & AF_X0) == 0, "the x = 0 form keeps march_rows' walk");
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long sy = nx, sz = (long)nx * ny;
+  const CT* __restrict__ kx = a.c.kx;
+  const CT* __restrict__ ky = a.c.ky;
+  const CT* __restrict__ kz = a.c.kz;
+  const CT* __restrict__ tx = a.c.tx;
+  const CT* __restrict__ ty = a.c.ty;
+  const CT* __restrict__ tz = a.c.tz;
+  const CT* __restrict__ gzz = a.c.gz;
+  const CT* __restrict__ vol = a.c.vol;
+  const CT* __restrict__ dut = a.c.dut;
+  const VX* __restrict__ X = a.x;
+  const float* __restrict__ DX = a.dx;
+  VO* __restrict__ Y = a.y;
+  VO* __restrict__ Y2 = a.y2;
+  const VU* __restrict__ U = a.u;
+  double* __restrict__ XW = a.xw;
+  const double* __restrict__ Bv = a.b;
+  const CT fa = a.fa;
+  if (L >= a.ntiles) return;
+  const int bx = L % a.gx, rest = L / a.gx, by = rest % a.gy, kt = rest / a.gy;
+  const int i = bx * a.bxw + threadIdx.x;
+  const int j0 = (by * a.byw + threadIdx.y) * R;
+  if (i >= nx || j0 >= ny) return;
+  const int k0 = kt * a.kp, k1 = min(nz, k0 + a.kp);
+  long cb[R];  // cell index of each row at level 0 (rows past ny: row ny-1)
+  bool ok[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    ok[r] = j0 + r < ny;
+    cb[r] = i + sy * (ok[r] ? j0 + r : ny - 1);
+  }
+  const long cbS = j0 > 0 ? cb[0] - sy : cb[0];
+  const long cbN = j0 + R < ny ? cb[R - 1] + sy : cb[R - 1];
+  const long dW = i > 0 ? -1 : 0, dE = i + 1 < nx ? 1 : 0;
+  auto LX = [&](long m) -> V2<CT> {
+    V2<VX> v = ld2(X, m);
+    V2<CT> r{(CT)v.a, (CT)v.b};
+    if constexpr ((FL & AF_DX) != 0) {
+      V2<float> e = ld2(DX, m);
+      r.a += (CT)e.a; r.b += (CT)e.b;
+    }
+    return r;
+  };
+  auto lev = [&](int k) -> long { return sz * (long)(k < nz ? k : nz - 1); };
+  struct Cells {
+    V2<CT> x[R], xw[R], xe[R], xS, xN;
+    CT d[R], v[R], dw[R], de[R], dS, dN;
+  };
+  struct Faces {
+    CT kz[R], tz[R], gz[R], ky[R], ty[R], kxw[R], kxe[R], txw[R], txe[R], kyS, tyS;
+    V2<VU> u[R];
+    double2 b[R];
+  };
+  auto ldcells = [&](int k, Cells& C) {
+    const long o = lev(k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long c = cb[r] + o;
+      C.x[r] = LX(c); C.d[r] = dut[c]; C.v[r] = ld_once(vol, c);
+      C.xw[r] = LX(c + dW); C.xe[r] = LX(c + dE); C.dw[r] = 0; C.de[r] = 0;
+      if constexpr (TR == 0) { C.dw[r] = dut[c + dW]; C.de[r] = dut[c + dE]; }
+    }
+    C.xS = LX(cbS + o); C.xN = LX(cbN + o); C.dS = 0; C.dN = 0;
+    if constexpr (TR == 0) { C.dS = dut[cbS + o]; C.dN = dut[cbN + o]; }
+  };
+  auto ldfaces = [&](int k, Faces& F) {
+    const long o = lev(k);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long c = cb[r] + o;
+      F.kz[r] = ld_once(kz, c); F.tz[r] = ld_once(tz, c); F.gz[r] = ld_once(gzz, c);
+      F.ky[r] = ky[c]; F.ty[r] = ty[c];
+      F.kxw[r] = kx[c + dW]; F.kxe[r] = kx[c]; F.txw[r] = tx[c + dW]; F.txe[r] = tx[c];
+      F.u[r] = V2<VU>{0, 0};
+      if constexpr ((FL & AF_YU) != 0) F.u[r] = ld2(U, c);
+      F.b[r] = make_double2(0.0, 0.0);
+      if constexpr ((FL & AF_B) != 0) F.b[r] = *reinterpret_cast<const double2*>(Bv + 2 * c);
+    }
+    F.kyS = ky[cbS + o]; F.tyS = ty[cbS + o];
+  };
+  V2<CT> xm[R];
+  CT dm[R], vm[R], kzm[R], tzm[R], gzm[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    xm[r] = V2<CT>{0, 0}; dm[r] = 0; vm[r] = 0; kzm[r] = 0; tzm[r] = 0; gzm[r] = 0;
+    if (k0 > 0) {
+      const long m = cb[r] + sz * (k0 - 1);
+      xm[r] = LX(m); dm[r] = dut[m]; vm[r] = ld_once(vol, m); kzm[r] = ld_once(kz, m);
+      tzm[r] = ld_once(tz, m); gzm[r] = ld_once(gzz, m);
+    }
+  }
+  Cells C0, C1, C2;
+  Faces F0, F1;
+  ldcells(k0, C0);
+  ldcells(k0 + 1, C1);
+  ldfaces(k0, F0);
+  for (int k = k0; k < k1; ++k) {
+    ldcells(k + 2, C2);
+    ldfaces(k + 1, F1);
+    const long off = sz * k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const long c = cb[r] + off;
+      const V2<CT> x0 = C0.x[r], xp = C1.x[r];
+      const CT d0 = C0.d[r], v0 = C0.v[r], dp = C1.d[r], vp = C1.v[r];
+      const CT kz0 = F0.kz[r], tz0 = F0.tz[r], gz0 = F0.gz[r];
+      const V2<CT> xW = C0.xw[r], xE = C0.xe[r];
+      const V2<CT> xs = r > 0 ? C0.x[r - 1] : C0.xS, xn = r + 1 < R ? C0.x[r + 1] : C0.xN;
+      const CT ds = r > 0 ? C0.d[r - 1] : C0.dS, dn = r + 1 < R ? C0.d[r + 1] : C0.dN;
+      const CT kys = r > 0 ? F0.ky[r - 1] : F0.kyS, tys = r > 0 ? F0.ty[r - 1] : F0.tyS;
+      const V2<VU> u0 = F0.u[r];
+      const double2 b0 = F0.b[r];
+      CT s1 = fa * v0 * x0.a, s2 = v0 * x0.b;
+      if constexpr (TR == 0) {
+        const CT q0 = d0 * x0.a;
+        auto hf = [&](CT kk, CT th, CT dnb, CT xnb) {
+          const CT t = kk * (q0 - dnb * xnb);
+          s1 += th * t; s2 += t;
+        };
+        hf(F0.kxw[r], F0.txw[r], C0.dw[r], xW.a);
+        hf(F0.kxe[r], F0.txe[r], C0.de[r], xE.a);
+        hf(kys, tys, ds, xs.a);
+        hf(F0.ky[r], F0.ty[r], dn, xn.a);
+        {
+          const CT t = kzm[r] * (q0 - dm[r] * xm[r].a);
+          const CT B = gzm[r] * (vm[r] * xm[r].b + v0 * x0.b);
+          s1 += tzm[r] * (t + B); s2 += t + B;
+        }
+        {
+          const CT t = kz0 * (q0 - dp * xp.a);
+          const CT B = gz0 * (v0 * x0.b + vp * xp.b);
+          s1 += tz0 * (t - B); s2 += t - B;
+        }
+      } else {
+        CT h = 0;
+        const V2<CT> xc = x0;
+        auto hf = [&](CT kk, CT th, V2<CT> xnb) {
+          h += kk * ((th * xc.a + xc.b) - (th * xnb.a + xnb.b));
+        };
+        hf(F0.kxw[r], F0.txw[r], xW);
+        hf(F0.kxe[r], F0.txe[r], xE);
+        hf(kys, tys, xs);
+        hf(F0.ky[r], F0.ty[r], xn);
+        {
+          const CT zc = tzm[r] * xc.a + xc.b, zn = tzm[r] * xm[r].a + xm[r].b;
+          h += kzm[r] * (zc - zn);
+          s2 += gzm[r] * v0 * (zc - zn);
+        }
+        {
+          const CT zc = tz0 * xc.a + xc.b, zn = tz0 * xp.a + xp.b;
+          h += kz0 * (zc - zn);
+          s2 += gz0 * v0 * (zn - zc);
+        }
+        s1 += d0 * h;
+      }
+      double yv1, yv2;
+      if constexpr ((FL & AF_B) != 0) {
+        const bool dead = v0 == (CT)0 &&
+                          F0.kxw[r] + F0.kxe[r] + kys + F0.ky[r] + kzm[r] + kz0 == (CT)0;
+        yv1 = dead ? 0.0 : b0.x + (double)s1;
+        yv2 = dead ? 0.0 : b0.y + (double)s2;
+      } else {
+        yv1 = -(double)s1; yv2 = -(double)s2;
+      }
+      if (ok[r]) {
+        if constexpr ((FL & AF_DX) != 0) st2(XW, c, (double)x0.a, (double)x0.b);
+        st2(Y, c, yv1, yv2);
+        if constexpr ((FL & AF_Y2) != 0) st2(Y2, c, yv1, yv2);
+        if constexpr ((FL & AF_YY) != 0) acc[0] += yv1 * yv1 + yv2 * yv2;
+        if constexpr ((FL & AF_YU) != 0) acc[1] += yv1 * (double)u0.a + yv2 * (double)u0.b;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      xm[r] = C0.x[r]; dm[r] = C0.d[r]; vm[r] = C0.v[r];
+      kzm[r] = F0.kz[r]; tzm[r] = F0.tz[r]; gzm[r] = F0.gz[r];
+    }
+    C0 = C1;
+    C1 = C2;
+    F0 = F1;
+  }
+}
+
 }  // namespace asam

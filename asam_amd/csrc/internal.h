@@ -51,6 +51,7 @@ struct Tuning {
                           // last workgroup; 2: the BiCGStab iteration's tails folded into
                           // the kernels that consume their scalars (0: a separate finalize
                           // launch each)
+  int apply_depth = 2;    // variant 3, FP32 2x2 blocks, two rows: levels of loads ahead (1, 2)
   int line_nt = 2;        // line solve's factor loads (and the vectors it reads once):
                           // 0 plain, 1 nontemporal, 2 nontemporal when the factors of the
                           // solve's precision exceed kLineNtBytes

@@ -134,6 +134,7 @@ extern "C" int asam_sound_get_tuning(const asam_sound_ctx* c, asam_sound_tuning*
   t->apply_band_rows = c->g.tune.apply_band_rows;
   t->apply_rows = c->g.tune.apply_rows;
   t->line_nt = c->g.tune.line_nt;
+  t->apply_depth = c->g.tune.apply_depth;
   return 0;
 }
 
@@ -142,6 +143,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   if (t->apply_variant < 0 || t->apply_variant > 3) return fail(c, ASAM_E_INVALID, "apply_variant");
   if (t->apply_rows != 2 && t->apply_rows != 4) return fail(c, ASAM_E_INVALID, "apply_rows");
   if (t->line_nt < 0 || t->line_nt > 2) return fail(c, ASAM_E_INVALID, "line_nt");
+  if (t->apply_depth != 1 && t->apply_depth != 2) return fail(c, ASAM_E_INVALID, "apply_depth");
   if (t->apply_band_rows < 0) return fail(c, ASAM_E_INVALID, "apply_band_rows");
   if (t->apply_wg < 256 || t->apply_wg > 16384) return fail(c, ASAM_E_INVALID, "apply_wg");
   if (t->line_window != 0 && t->line_window != 1 && t->line_window != 4 && t->line_window != 8)
@@ -162,6 +164,7 @@ extern "C" int asam_sound_set_tuning(asam_sound_ctx* c, const asam_sound_tuning*
   nt.apply_band_rows = t->apply_band_rows;
   nt.apply_rows = t->apply_rows;
   nt.line_nt = t->line_nt;
+  nt.apply_depth = t->apply_depth;
   Geom tg = c->g;
   tg.tune = nt;
   if (apply_blocks(tg, 0) > kRedBlocks || apply_blocks(tg, 1) > kRedBlocks)
@@ -1486,14 +1489,18 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
   // loaded by one wavefront in one iteration
   // (fewer workgroups than the other variants: longer z-marches per tile; the best count
   // depends on the grid and is not monotonic)
+  // (loads two levels ahead: FP32 2x2 blocks only)
   const int rbs[] = {2, 4}, rbxs[] = {32, 64, 128, 256}, rwgs[] = {256, 512, 1024, 2048, 4096};
+  const int depths = (prec == 1 && c->g.np1 == 2) ? 2 : 1;
+  for (int dp = 1; dp <= depths; ++dp)
   for (int rr : rbs)
     for (int bx : rbxs)
       for (int xcd : xcds)
         for (int wg : rwgs) {
           c->g.tune = bt;
           c->g.tune.apply_variant = 3; c->g.tune.apply_rows = rr; c->g.tune.apply_bx = bx;
-          c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg;
+          c->g.tune.apply_xcd = xcd; c->g.tune.apply_wg = wg; c->g.tune.apply_depth = dp;
+          if (dp == 2 && rr == 4) continue;  // two rows only
           if (apply_blocks(c->g, 0) > kRedBlocks || apply_blocks(c->g, 1) > kRedBlocks) continue;
           double m = 0;
           int rc = measure(&m);

@@ -49,7 +49,8 @@ class Tuning(C.Structure):
     _fields_ = [("apply_variant", C.c_int), ("apply_wg", C.c_int), ("line_window", C.c_int),
                 ("line_lds", C.c_int), ("fused_tail", C.c_int), ("apply_xcd", C.c_int),
                 ("apply_bx", C.c_int), ("line_tiled", C.c_int), ("apply_band_rows", C.c_int),
-                ("apply_rows", C.c_int), ("line_nt", C.c_int)]
+                ("apply_rows", C.c_int), ("line_nt", C.c_int),
+                ("apply_depth", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -120,6 +120,9 @@ typedef struct {
                         reads once): 0 plain, 1 nontemporal, 2 (default) nontemporal when the
                         factors of the solve's precision exceed 64 MiB, plain below, where
                         they stay resident in the Infinity Cache.  Same bits either way */
+  int apply_depth;   /* variant 3 with FP32 coefficients, 2x2 blocks and two rows: the
+                        levels a thread loads ahead of the one it computes, 1 or 2
+                        (default).  Same bits */
 } asam_sound_tuning;
 
 /* ---- context ---------------------------------------------------------- */

@@ -457,6 +457,38 @@ def test_apply_variants_keep_parity(np1, oracle_mod, sound_mod):
         assert im.status == 0 and rr <= 1.0001e-8
 
 
+@pytest.mark.parametrize("transpose", [False, True])
+@pytest.mark.parametrize("dims", [(70, 37, 11), (9, 1, 6), (5, 3, 4), (66, 5, 3)])
+def test_deep_apply_same_bits(dims, transpose, oracle_mod, sound_mod):
+    """The row-blocked apply with its loads two levels ahead (apply_depth = 2, used by the
+    FP32 applies of the mixed solve with 2x2 blocks) does march_rows' arithmetic: the mixed
+    solve gives the same bits at every width, XCD order and z-tiling (one tile of all
+    levels, and tiles of a few levels), forward and transposed, on grids down to one row
+    and fewer levels than the look-ahead."""
+    p = small(2, nx=dims[0], ny=dims[1], nz=dims[2])
+    orc = oracle_mod.Oracle(p)
+    s = sound_mod.SoundSolver.from_problem(p)
+    s.set_control(precision="MIXED")
+    solve = s.SolveSound_T if transpose else s.SolveSound
+    for bx in (32, 64, 128, 256):
+        for wg, xcd in ((256, 1), (4096, 0)):
+            out = []
+            for depth in (1, 2):
+                s.set_tuning(apply_variant=3, apply_rows=2, apply_bx=bx, apply_wg=wg,
+                             apply_xcd=xcd, apply_depth=depth)
+                out.append(solve(np.zeros(p.n), p.b, 300, 1e-8))
+            (xa, ma, ta, ia), (xb, mb, tb, ib) = out
+            op = orc.matvec1_T if transpose else orc.matvec1
+            rr = np.linalg.norm(p.b - op(xb)) / np.linalg.norm(p.b)
+            print(f"{dims} transpose={transpose} bx {bx} wg {wg} xcd {xcd}: max|dx| "
+                  f"{np.abs(xa - xb).max():.1e}, inner {ia.iterations}/{ib.iterations}, "
+                  f"true residual {rr:.2e}")
+            assert ib.status == 0 and rr <= 1.0001e-8
+            assert np.array_equal(xa, xb)
+            assert (ia.iterations, ia.restarts, ia.abs_residual) == \
+                (ib.iterations, ib.restarts, ib.abs_residual)
+
+
 @pytest.mark.parametrize("np1", [1, 2])
 @pytest.mark.parametrize("dims", [(9, 1, 6), (5, 3, 4), (66, 5, 3)])
 def test_row_blocked_apply_thin_grids(np1, dims, oracle_mod, sound_mod):
