@@ -70,8 +70,9 @@ __device__ __forceinline__ V2<T> mtv(const M4<T>& q, V2<T> v) {  // q^T * v
 //
 // NT: cache policy of the factor loads, of the two vectors only this kernel reads (p in
 // the p-step, v in the s-step) and of the p-step's store of p (read again only by the next
-// iteration's p-step), see ld4.
-template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT, int NT = 1>
+// iteration's p-step), see ld4.  PBW: levels per window of the backward sweep (0: P); the
+// backward sweep runs after the whole forward sweep, its partial window at the top first.
+template <int NP, int TR, int MODE, int LDS, int P, typename CT, typename VT, int NT = 1, int PBW = 0>
 __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j, int gxl,
                                           CT* stage, double (&acc)[1]) {
   const int nx = a.nx, nz = a.nz;
@@ -224,12 +225,13 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
     };
     // --- backward sweep: z_k = y_k - w_k z_k+1   (transposed: t_k^-T (y_k - l_k+1^T z_k+1))
     V znext{0, 0};
+    constexpr int PB = PBW > 0 ? PBW : P;
     auto bwd = [&](int kb, auto chk) {
       constexpr bool CHK = decltype(chk)::value;
-      V y_[P];
-      M w_[P], t_[P];
+      V y_[PB];
+      M w_[PB], t_[PB];
 #pragma unroll
-      for (int q = 0; q < P; ++q) {
+      for (int q = 0; q < PB; ++q) {
         const int k = kb + q;
         if (!CHK || k < nz) {
           const long c = c0 + sz * k;
@@ -243,7 +245,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
         }
       }
 #pragma unroll
-      for (int q = P - 1; q >= 0; --q) {
+      for (int q = PB - 1; q >= 0; --q) {
         const int k = kb + q;
         if (!CHK || k < nz) {
           const long c = c0 + sz * k;
@@ -262,11 +264,10 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
     };
     const int nfull = nz / P;
     for (int w = 0; w < nfull; ++w) fwd(w * P, std::false_type{});
-    if (nfull * P < nz) {
-      fwd(nfull * P, std::true_type{});
-      bwd(nfull * P, std::true_type{});
-    }
-    for (int w = nfull - 1; w >= 0; --w) bwd(w * P, std::false_type{});
+    if (nfull * P < nz) fwd(nfull * P, std::true_type{});
+    const int nfb = nz / PB;
+    if (nfb * PB < nz) bwd(nfb * PB, std::true_type{});
+    for (int w = nfb - 1; w >= 0; --w) bwd(w * PB, std::false_type{});
   }
 }
 
