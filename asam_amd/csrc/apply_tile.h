@@ -270,6 +270,10 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
 template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
 __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int L,
                                            double (&acc)[2]) {
+  // multiply-adds fused within a statement only: the bits follow the source, so
+  // march_rows_deep (the same statements with another order of loads) gives the same
+  // values (hipcc's default also fuses across statements, which depends on code shape)
+#pragma clang fp contract(on)
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long sy = nx, sz = (long)nx * ny;
   const CT* __restrict__ kx = a.c.kx;
@@ -560,6 +564,7 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
 template <int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
 __device__ __forceinline__ void march_rows_deep(const KApply<CT, VX, VO, VU>& a, int L,
                                                 double (&acc)[2]) {
+#pragma clang fp contract(on)  // as in march_rows
   static_assert((FL & AF_X0) == 0, "the x = 0 form keeps march_rows' walk");
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long sy = nx, sz = (long)nx * ny;
