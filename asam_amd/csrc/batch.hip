@@ -70,7 +70,8 @@ struct KEnsApply {
 };
 
 // PAR: refinement apply x_eff = X[(o-1)&1] + d written to X[o&1], o = refinement step.
-template <int NP, int FL, int PAR, typename CT, typename VX, typename VO, typename VU>
+// RB: rows per thread (1: march_tile, 2: the row-blocked march_rows).
+template <int NP, int FL, int PAR, int RB, typename CT, typename VX, typename VO, typename VU>
 __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) {
   if (e.skip && *e.skip) return;
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
@@ -110,7 +111,8 @@ __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) 
     a.done = nullptr;
     a.dead = e.dead ? e.dead + M.off : nullptr;
     double acc[2] = {0.0, 0.0};
-    march_tile<NP, 0, FL, CT, VX, VO, VU>(a, g - M.at0, acc);
+    if constexpr (RB == 2) march_rows<NP, 0, FL, 2, CT, VX, VO, VU>(a, g - M.at0, acc);
+    else march_tile<NP, 0, FL, CT, VX, VO, VU>(a, g - M.at0, acc);
     if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
       block_reduce_store<2>(acc, e.partial, g, tid, 256, e.pstride);
       __syncthreads();  // the reduction scratch is reused by the next tile
@@ -437,6 +439,7 @@ static int grid_for(int ntiles, int pf, int ncu) {
 static int valid_tuning(const asam_ens_tuning& t) {
   if (t.apply_bx != 64 && t.apply_bx != 128 && t.apply_bx != 256) return 0;
   if (t.apply_zsplit < 1 || t.apply_zsplit > 8) return 0;
+  if (t.apply_rows < 0 || t.apply_rows > 2) return 0;
   if (t.apply_pf < 0 || t.apply_pf > 64 || t.line_pf < 0 || t.line_pf > 64) return 0;
   if (t.blas_pf < 0 || t.blas_pf > 64) return 0;
   if (t.line_window != 4 && t.line_window != 8) return 0;
@@ -447,7 +450,8 @@ static int valid_tuning(const asam_ens_tuning& t) {
 
 // Tile tables of the current tuning (device copies), partial-sum stride.
 static int build_tables(asam_ens_ctx* c) {
-  const int bx = c->tune.apply_bx, by = 256 / bx, chunk = c->tune.blas_chunk;
+  const int bx = c->tune.apply_bx, chunk = c->tune.blas_chunk;
+  const int by = (256 / bx) * (c->tune.apply_rows == 2 ? 2 : 1);  // rows per apply tile
   long gA = 0, gL = 0, gB = 0;
   for (int m = 0; m < c->nmem; ++m) {
     EnsMember& M = c->mh[m];
@@ -519,6 +523,17 @@ void fin(asam_ens_ctx* c, int kind, int nq, int srcs) {
                      (const double*)c->partial, c->pstride, (const EnsMember*)c->md, c->st);
 }
 
+template <int NP, int FL, int PAR, typename CT, typename VX, typename VO, typename VU>
+static void ens_apply_go(asam_ens_ctx* c, dim3 grid, dim3 blk,
+                         const KEnsApply<CT, VX, VO, VU>& e) {
+  if (c->tune.apply_rows == 2)
+    hipLaunchKernelGGL((k_ens_apply<NP, FL, PAR, 2, CT, VX, VO, VU>), grid, blk, 0, c->stream,
+                       e);
+  else
+    hipLaunchKernelGGL((k_ens_apply<NP, FL, PAR, 1, CT, VX, VO, VU>), grid, blk, 0, c->stream,
+                       e);
+}
+
 template <int NP>
 void apply_outer(asam_ens_ctx* c, bool first, double* x, const double* b) {
   KEnsApply<double, double, float, float> e;
@@ -533,17 +548,13 @@ void apply_outer(asam_ens_ctx* c, bool first, double* x, const double* b) {
     // cells: the same values and partial sums from a quarter of the bytes)
     launch_any_nonzero(c->stream, c->n, x, c->xflags, c->xflags + 1);
     e.skip = c->xflags + 1;
-    hipLaunchKernelGGL((k_ens_apply<NP, AF_B | AF_Y2 | AF_YY, 0, double, double, float, float>),
-                       grid, blk, 0, c->stream, e);
+    ens_apply_go<NP, AF_B | AF_Y2 | AF_YY, 0>(c, grid, blk, e);
     e.skip = c->xflags;
     e.dead = c->dead;
-    hipLaunchKernelGGL(
-        (k_ens_apply<NP, AF_X0 | AF_B | AF_Y2 | AF_YY, 0, double, double, float, float>), grid,
-        blk, 0, c->stream, e);
-  } else
-    hipLaunchKernelGGL(
-        (k_ens_apply<NP, AF_B | AF_Y2 | AF_DX | AF_YY, 1, double, double, float, float>), grid,
-        blk, 0, c->stream, e);
+    ens_apply_go<NP, AF_X0 | AF_B | AF_Y2 | AF_YY, 0>(c, grid, blk, e);
+  } else {
+    ens_apply_go<NP, AF_B | AF_Y2 | AF_DX | AF_YY, 1>(c, grid, blk, e);
+  }
 }
 
 template <int NP, int FL>
@@ -552,9 +563,8 @@ void apply_inner(asam_ens_ctx* c, float* x, float* y, const float* u) {
   e.c = c->c32; e.mem = c->md; e.tmem = c->tA; e.ntiles = c->nA; e.bxw = c->tune.apply_bx;
   e.xa = x; e.xb = nullptr; e.dx = nullptr; e.b = nullptr; e.y = y; e.y2 = nullptr; e.u = u;
   e.partial = c->partial; e.pstride = c->pstride; e.st = c->st; e.skip_outer = 0;
-  hipLaunchKernelGGL((k_ens_apply<NP, FL, 0, float, float, float, float>),
-                     dim3(grid_for(c->nA, c->tune.apply_pf, c->ncu)),
-                     dim3(c->tune.apply_bx, 256 / c->tune.apply_bx), 0, c->stream, e);
+  ens_apply_go<NP, FL, 0>(c, dim3(grid_for(c->nA, c->tune.apply_pf, c->ncu)),
+                         dim3(c->tune.apply_bx, 256 / c->tune.apply_bx), e);
 }
 
 template <int NP, int MODE, int LDS, int P>
@@ -780,6 +790,7 @@ extern "C" int asam_ens_create(asam_ens_ctx** out, int nmem, const asam_sound_gr
   asam_sound_default_control(&c->ctl);
   c->ctl.precision = ASAM_PREC_MIXED;
   c->tune.apply_bx = 256; c->tune.apply_zsplit = 1; c->tune.apply_pf = 0;
+  c->tune.apply_rows = 2;
   c->tune.line_window = 8; c->tune.line_lds = 1; c->tune.line_pf = 0;
   c->tune.blas_chunk = 4096; c->tune.blas_pf = 0; c->tune.use_graph = 1;
   c->sched = {10, 6, 4};
@@ -1070,13 +1081,15 @@ int autotune_np(asam_ens_ctx* c, int reps, double* best_ms) {
   double bms = 1e30;
   int rc;
   {
-    const int bxs[] = {64, 128, 256}, zss[] = {1, 2, 4}, pfs[] = {0, 2, 8};
+    const int bxs[] = {64, 128, 256}, zss[] = {1, 2, 4}, pfs[] = {0, 2, 8}, rbs[] = {1, 2};
     const asam_ens_tuning t0 = bt;
+    for (int rb : rbs)
     for (int bx : bxs)
       for (int zs : zss)
         for (int pf : pfs) {
           c->tune = t0;
           c->tune.apply_bx = bx; c->tune.apply_zsplit = zs; c->tune.apply_pf = pf;
+          c->tune.apply_rows = rb;
           double ms = 0;
           if ((rc = measure(&ms))) return rc;
           if (ms < bms) { bms = ms; bt = c->tune; }

@@ -21,7 +21,7 @@ class EnsTuning(C.Structure):
     _fields_ = [("apply_bx", C.c_int), ("apply_zsplit", C.c_int), ("apply_pf", C.c_int),
                 ("line_window", C.c_int), ("line_lds", C.c_int), ("line_pf", C.c_int),
                 ("blas_chunk", C.c_int), ("blas_pf", C.c_int), ("use_graph", C.c_int),
-                ("reserved", C.c_int * 3)]
+                ("apply_rows", C.c_int), ("reserved", C.c_int * 2)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

@@ -41,7 +41,9 @@ typedef struct {
   int blas_chunk;    /* vector elements per tile of the update kernels: 1024..65536 */
   int blas_pf;       /* workgroups per CU of the persistent update grid; 0: one per tile */
   int use_graph;     /* 1: capture the solve schedule into a HIP graph (default) */
-  int reserved[3];
+  int apply_rows;    /* rows of a column each apply thread walks: 2 (default, the
+                        neighbouring row from registers) or 1 (0 reads as 1) */
+  int reserved[2];
 } asam_ens_tuning;
 
 /* Per-ensemble counters of the last solve. */

@@ -27,13 +27,16 @@ def ens_mod(sound_mod):
     return ensemble
 
 
+@pytest.mark.parametrize("rows", [1, 2])
 @pytest.mark.parametrize("np1", [1, 2])
-def test_ensemble_mixed_reaches_1e8_and_matches_fp64_oracle(np1, ens_mod, oracle_mod):
+def test_ensemble_mixed_reaches_1e8_and_matches_fp64_oracle(np1, rows, ens_mod, oracle_mod):
+    """Both apply walks: one row per thread, and two (the default)."""
     ps = members(np1)
     E = ens_mod.Ensemble(ps)
+    E.set_tuning(apply_rows=rows)
     x, infos = E.solve([p.b for p in ps], tol=1e-8)
     st = E.stats()
-    print(f"np1={np1}: stats {st}")
+    print(f"np1={np1} rows={rows}: stats {st}")
     for m, (p, xm, inf) in enumerate(zip(ps, E.split(x), infos)):
         orc = oracle_mod.Oracle(p)
         rr = rel_residual(orc, xm, p.b)
