@@ -151,7 +151,7 @@ __global__ __launch_bounds__(64) void k_ens_line(KEnsLine<CT, VT> e) {
     KLine<CT, VT> a;
     a.c = e.c;
     const long fo = (long)NP * NP * M.foff;
-    a.c.pl += fo; a.c.pt += fo; a.c.pw += fo;
+    a.c.pl += lower_words(NP) * M.foff; a.c.pt += fo; a.c.pw += fo;
     a.nx = M.nx; a.ny = M.ny; a.nz = M.nz; a.tiled = 1;
     const long v = (long)NP * M.off;
     a.src = e.src ? e.src + v : nullptr;
@@ -422,7 +422,7 @@ static Coef<T> member_coef(const Coef<T>& pool, const EnsMember& M, int np1) {
   k.kx += M.off; k.ky += M.off; k.kz += M.off; k.vol += M.off; k.dut += M.off;
   if (np1 == 2) { k.tx += M.off; k.ty += M.off; k.tz += M.off; k.gz += M.off; }
   const long fo = (long)np1 * np1 * M.foff;
-  k.pl += fo; k.pt += fo; k.pw += fo;
+  k.pl += lower_words(np1) * M.foff; k.pt += fo; k.pw += fo;
   return k;
 }
 
@@ -803,7 +803,7 @@ extern "C" int asam_ens_create(asam_ens_ctx** out, int nmem, const asam_sound_gr
   EHIP(c, hipEventCreate(&c->ev0));
   EHIP(c, hipEventCreate(&c->ev1));
   (void)lds_budget();  // device query cached here, never during a graph capture
-  const long N = Ntot, P = (long)np1 * np1 * Ftot;
+  const long N = Ntot, P = (long)np1 * np1 * Ftot, PL = (long)lower_words(np1) * Ftot;
   int rc = 0;
   auto coef = [&](auto& k) {
     rc |= ealloc(c, &k.kx, N); rc |= ealloc(c, &k.ky, N); rc |= ealloc(c, &k.kz, N);
@@ -812,7 +812,7 @@ extern "C" int asam_ens_create(asam_ens_ctx** out, int nmem, const asam_sound_gr
       rc |= ealloc(c, &k.gz, N);
     }
     rc |= ealloc(c, &k.vol, N); rc |= ealloc(c, &k.dut, N);
-    rc |= ealloc(c, &k.pl, P); rc |= ealloc(c, &k.pt, P); rc |= ealloc(c, &k.pw, P);
+    rc |= ealloc(c, &k.pl, PL); rc |= ealloc(c, &k.pt, P); rc |= ealloc(c, &k.pw, P);
   };
   coef(c->c64);
   coef(c->c32);

@@ -80,6 +80,10 @@ __host__ __device__ inline long fac_index(const Geom& g, int i, int j, int k) {
   }
   return i + (long)g.nx * (j + (long)g.ny * k);
 }
+// Words per cell of the line solve's lower blocks L_k.  A 2x2 L_k couples level k to the
+// cell below through one face: L_k = (tz, 1)^T (-kz DUT, gz V) of that cell, rank one, so
+// it is stored as the three words (tz, -kz DUT, gz V).
+constexpr int lower_words(int np1) { return np1 == 2 ? 3 : 1; }
 // Entries per line-factor array (tiled layout pads x to a multiple of 64).
 inline long fac_cells(const Geom& g) { return ((long)(g.nx + 63) / 64) * 64 * g.ny * g.nz; }
 

@@ -140,6 +140,36 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
         return ld4<NT>(p, c);
       }
     };
+    // a lower block L: for 2x2 blocks the three words (tz, v0, v1) of
+    // L = (tz, 1)^T (v0, v1), in m[0..2]
+    auto ldl = [&](const CT* p, long c) -> M {
+      if constexpr (NP == 1) {
+        return ldm(p, c);
+      } else {
+        const CT* q = p + 3 * c;
+        if constexpr (NT)
+          return M{{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
+                    __builtin_nontemporal_load(q + 2), (CT)0}};
+        else
+          return M{{q[0], q[1], q[2], (CT)0}};
+      }
+    };
+    auto mull = [&](const M& l, V y) -> V {  // L y = (tz s, s), s = v0 y.a + v1 y.b
+      if constexpr (NP == 1) {
+        return V{l.m[0] * y.a, (CT)0};
+      } else {
+        const CT s = l.m[1] * y.a + l.m[2] * y.b;
+        return V{l.m[0] * s, s};
+      }
+    };
+    auto mullT = [&](const M& l, V z) -> V {  // L^T z = (v0 t, v1 t), t = tz z.a + z.b
+      if constexpr (NP == 1) {
+        return V{l.m[0] * z.a, (CT)0};
+      } else {
+        const CT t = l.m[0] * z.a + z.b;
+        return V{l.m[1] * t, l.m[2] * t};
+      }
+    };
     auto mul = [&](const M& m, V v) -> V {  // m * v
       if constexpr (NP == 1) return V{m.m[0] * v.a, (CT)0};
       else return mv(m, v);
@@ -189,7 +219,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
           }
           if constexpr (MODE == LINE_BICG_S) { r_[q] = ldv((const VT*)PP, c); v_[q] = ldvo(VV, c); }
           const long f = f0 + fs * k;
-          if constexpr (TR == 0) { l_[q] = ldm(PL, f); t_[q] = ldm(PT, f); }
+          if constexpr (TR == 0) { l_[q] = ldl(PL, f); t_[q] = ldm(PT, f); }
           else l_[q] = (k > 0) ? ldm(PW, f - fs) : M{{0, 0, 0, 0}};
         }
       }
@@ -212,7 +242,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
           }
           V y;
           if constexpr (TR == 0) {
-            const V ly = mul(l_[q], yprev);
+            const V ly = mull(l_[q], yprev);
             y = mul(t_[q], V{in.a - ly.a, in.b - ly.b});
           } else {
             const V wy = mulT(l_[q], yprev);
@@ -239,7 +269,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
           const long f = f0 + fs * k;
           if constexpr (TR == 0) w_[q] = ldm(PW, f);
           else {
-            w_[q] = (k + 1 < nz) ? ldm(PL, f + fs) : M{{0, 0, 0, 0}};
+            w_[q] = (k + 1 < nz) ? ldl(PL, f + fs) : M{{0, 0, 0, 0}};
             t_[q] = ldm(PT, f);
           }
         }
@@ -254,7 +284,7 @@ __device__ __forceinline__ void line_tile(const KLine<CT, VT>& a, int bxl, int j
             const V wz = mul(w_[q], znext);
             z = V{y_[q].a - wz.a, y_[q].b - wz.b};
           } else {
-            const V lz = mulT(w_[q], znext);
+            const V lz = mullT(w_[q], znext);
             z = mulT(t_[q], V{y_[q].a - lz.a, y_[q].b - lz.b});
           }
           stv(OUT, c, V{-z.a, -z.b});

@@ -206,9 +206,11 @@ __global__ void k_factor(Geom g, Coef<double> c) {
       if (j > 0) hface(c.ky[cc - sy], c.ty[cc - sy]);
       if (j + 1 < ny) hface(c.ky[cc], c.ty[cc]);
       double L[4] = {0, 0, 0, 0};
+      double lz_tz = 0.0;  // tz of the face below (L's first row is tz times its second)
       if (k > 0) {
         const long m = cc - sz;
         const double kz = c.kz[m], tz = c.tz[m], gz = c.gz[m];
+        lz_tz = tz;
         hface(kz, tz);
         d[2] += tz * gz * V;   // +th g~ V_c   (d(1,2) of the upper cell)
         d[3] += gz * V;
@@ -243,11 +245,14 @@ __global__ void k_factor(Geom g, Coef<double> c) {
       if (k + 1 < nz) mul22(T, U, Wn);
       const long fi = fac_index(g, i, j, k);
       for (int q = 0; q < 4; ++q) {
-        c.pl[4 * fi + q] = L[q];
         c.pt[4 * fi + q] = T[q];
         c.pw[4 * fi + q] = Wn[q];
         W[q] = Wn[q];
       }
+      // L = (tz, 1)^T (L[1], L[3]): the sweeps use the three words
+      c.pl[3 * fi + 0] = lz_tz;
+      c.pl[3 * fi + 1] = L[1];
+      c.pl[3 * fi + 2] = L[3];
     }
   }
 }
@@ -264,13 +269,14 @@ __global__ void k_cvt(long n, const double* a, float* b) {
 
 void launch_convert(hipStream_t s, const Geom& g, const Coef<double>& c64, Coef<float>& c32) {
   const long N = g.N, P = (long)g.np1 * g.np1 * fac_cells(g);
+  const long PL = (long)lower_words(g.np1) * fac_cells(g);
   auto cv = [&](const double* a, float* b, long n) {
     if (a && b) hipLaunchKernelGGL(k_cvt, dim3((n + 255) / 256), dim3(256), 0, s, n, a, b);
   };
   cv(c64.kx, c32.kx, N); cv(c64.ky, c32.ky, N); cv(c64.kz, c32.kz, N);
   cv(c64.tx, c32.tx, N); cv(c64.ty, c32.ty, N); cv(c64.tz, c32.tz, N);
   cv(c64.gz, c32.gz, N); cv(c64.vol, c32.vol, N); cv(c64.dut, c32.dut, N);
-  cv(c64.pl, c32.pl, P); cv(c64.pt, c32.pt, P); cv(c64.pw, c32.pw, P);
+  cv(c64.pl, c32.pl, PL); cv(c64.pt, c32.pt, P); cv(c64.pw, c32.pw, P);
 }
 
 // Dead cells of the residual form: the apply's own test (march_tile), same operand order.

@@ -215,6 +215,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
   HIPCHK(c, hipEventCreate(&c->ev0));
   HIPCHK(c, hipEventCreate(&c->ev1));
   const long P = (long)grid->np1 * grid->np1 * fac_cells(c->g);
+  const long PL = (long)lower_words(grid->np1) * fac_cells(c->g);
   auto coef64 = [&](Coef<double>& k) -> int {
     int r = 0;
     r |= dalloc(c, &k.kx, N); r |= dalloc(c, &k.ky, N); r |= dalloc(c, &k.kz, N);
@@ -223,7 +224,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
       r |= dalloc(c, &k.gz, N);
     }
     r |= dalloc(c, &k.vol, N); r |= dalloc(c, &k.dut, N);
-    r |= dalloc(c, &k.pl, P); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
+    r |= dalloc(c, &k.pl, PL); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
     return r;
   };
   auto coef32 = [&](Coef<float>& k) -> int {
@@ -234,7 +235,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
       r |= dalloc(c, &k.gz, N);
     }
     r |= dalloc(c, &k.vol, N); r |= dalloc(c, &k.dut, N);
-    r |= dalloc(c, &k.pl, P); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
+    r |= dalloc(c, &k.pl, PL); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);
     return r;
   };
   rc |= coef64(c->c64);
@@ -416,7 +417,8 @@ double line_bytes(const Geom& g, const LineReq& r, bool first) {
   const double N = (double)g.N, NP = g.np1;
   const double wc = r.prec ? 4 : 8, wv = r.vt ? 4 : 8;
   double vec = r.mode == LINE_PLAIN ? 2 : (r.mode == LINE_BICG_P ? (first ? 3 : 5) : 4);
-  return N * 3 * NP * NP * wc + N * NP * wv * vec;
+  // factors: T^-1 and W (NP x NP words each) and L (lower_words) per cell
+  return N * (2 * NP * NP + lower_words(g.np1)) * wc + N * NP * wv * vec;
 }
 
 struct Enq {

@@ -392,7 +392,7 @@ def ensemble_bench(a):
     # p-step (5 vectors) + apply v (u) + line s-step (4 vectors) + apply t (u) + update
     np1 = a.np1
     coefw = 5 if np1 == 1 else 9
-    fac = 3 * np1 * np1 * 4
+    fac = (2 * np1 * np1 + (3 if np1 == 2 else 1)) * 4  # T^-1, W and the rank-one L
     it_bytes = cells * ((fac + np1 * 4 * 5) + (coefw * 4 + np1 * 12) + (fac + np1 * 4 * 4)
                         + (coefw * 4 + np1 * 12) + 8 * np1 * 4)
     gbs = it_bytes / (tms[0] * 1e-3) / 1e9
