@@ -116,8 +116,9 @@ struct KState {
 // Scalar tail of a reduction run inside a kernel in place of a finalize launch (kind < 0:
 // none): by the last workgroup of the kernel that produced the partial sums (fused tail),
 // or by every workgroup of the kernel that needs the scalars (folded tail).  The kinds are
-// those whose step reads and writes only the state's scalars ahead of H.  counter counts
-// the arriving workgroups; it is zero between launches (the last workgroup resets it).
+// those whose step reads and writes only the state's scalars ahead of H.  counter (fused
+// tails only) counts the arriving workgroups; it is zero between launches (the last
+// workgroup resets it).
 struct KTail {
   int kind = -1;
   int nq = 0, nblk = 0, nblk_hi = 0;  // slots, partials of slots 0-1 and of slots 2-3
