@@ -295,8 +295,9 @@ __global__ __launch_bounds__(256) void k_apply_march(KApply<CT, VX, VO, VU> a) {
   }
 }
 
-// Row-blocked z-march variant: R rows per thread (march_rows; DEEP: march_rows_deep, loads
-// two levels ahead), tiles of an XCD contiguous.
+// Row-blocked z-march variant: R rows per thread (march_rows; DEEP 1: march_rows_deep, loads
+// two levels ahead; DEEP 2: march_rows reading the packed z-face coefficients and volume),
+// tiles of an XCD contiguous.
 template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU,
           int DEEP = 0>
 __global__ __launch_bounds__(256) void k_apply_rows(KApply<CT, VX, VO, VU> a) {
@@ -305,7 +306,8 @@ __global__ __launch_bounds__(256) void k_apply_rows(KApply<CT, VX, VO, VU> a) {
   const int bphys = blockIdx.x, nb8 = gridDim.x >> 3;
   const int L = a.xcd ? (bphys & 7) * nb8 + (bphys >> 3) : bphys;
   double acc[2] = {0.0, 0.0};
-  if constexpr (DEEP) march_rows_deep<TR, FL, R, CT, VX, VO, VU>(a, L, acc);
+  if constexpr (DEEP == 1) march_rows_deep<TR, FL, R, CT, VX, VO, VU>(a, L, acc);
+  else if constexpr (DEEP == 2) march_rows<NP, TR, FL, R, CT, VX, VO, VU, 1>(a, L, acc);
   else march_rows<NP, TR, FL, R, CT, VX, VO, VU>(a, L, acc);
   if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
     block_reduce_store<2>(acc, a.partial, bphys, tid, 256);
@@ -341,6 +343,16 @@ static void apply_t(hipStream_t s, const Geom& g, const Coef<CT>& c, const Apply
       if (g.tune.apply_depth == 2 && !four) {
         hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 2, CT, VX, VO, VU, 1>), dim3(sh.nblocks),
                            dim3(a.bxw, a.byw), 0, s, a);
+        return;
+      }
+      // one level ahead: the z-face coefficients and volume from their packed copy
+      if (c.pkz) {
+        if (four)
+          hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 4, CT, VX, VO, VU, 2>), dim3(sh.nblocks),
+                             dim3(a.bxw, a.byw), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_apply_rows<NP, TR, FL, 2, CT, VX, VO, VU, 2>), dim3(sh.nblocks),
+                             dim3(a.bxw, a.byw), 0, s, a);
         return;
       }
     }

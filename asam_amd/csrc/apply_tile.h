@@ -267,7 +267,11 @@ __device__ __forceinline__ void march_tile(const KApply<CT, VX, VO, VU>& a, int 
 // below and above the block are loaded in the iteration that loads the cell above (the
 // owners of those cells load them then too) and carried one level, so they are not
 // re-read from the caches an iteration later.
-template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU>
+// ZP (2x2 blocks): the z-face coefficients and the volume, which no neighbour reads, come
+// from one nontemporal 16-byte load per cell of their packed copy (Coef::pkz), loaded with
+// the level above and carried: four streams in one.
+template <int NP, int TR, int FL, int R, typename CT, typename VX, typename VO, typename VU,
+          int ZP = 0>
 __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int L,
                                            double (&acc)[2]) {
   // multiply-adds fused within a statement only: the bits follow the source, so
@@ -436,16 +440,32 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
     };
     V2<CT> xm[R], x0[R], xw0[R], xe0[R];
     CT dm[R], vm[R], kzm[R], tzm[R], gzm[R], d0[R], v0[R], dw0[R], de0[R];
+    // ZP: the z-face coefficients of the current level, loaded with the level's volume
+    CT kzc[R], tzc[R], gzc[R];
+    typedef CT v4 __attribute__((ext_vector_type(4)));
+    const v4* __restrict__ PKZ = reinterpret_cast<const v4*>(a.c.pkz);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const long c = cb[r] + sz * k0;
       xm[r] = V2<CT>{0, 0}; dm[r] = 0; vm[r] = 0; kzm[r] = 0; tzm[r] = 0; gzm[r] = 0;
       if (k0 > 0) {
         const long m = c - sz;
-        xm[r] = LX(m); dm[r] = dut[m]; vm[r] = ld_once(vol, m); kzm[r] = ld_once(kz, m);
-        tzm[r] = ld_once(tz, m); gzm[r] = ld_once(gzz, m);
+        xm[r] = LX(m); dm[r] = dut[m];
+        if constexpr (ZP) {
+          const v4 q = __builtin_nontemporal_load(PKZ + m);
+          kzm[r] = q.x; tzm[r] = q.y; gzm[r] = q.z; vm[r] = q.w;
+        } else {
+          vm[r] = ld_once(vol, m); kzm[r] = ld_once(kz, m);
+          tzm[r] = ld_once(tz, m); gzm[r] = ld_once(gzz, m);
+        }
       }
-      x0[r] = LX(c); d0[r] = dut[c]; v0[r] = ld_once(vol, c);
+      x0[r] = LX(c); d0[r] = dut[c];
+      if constexpr (ZP) {
+        const v4 q = __builtin_nontemporal_load(PKZ + c);
+        kzc[r] = q.x; tzc[r] = q.y; gzc[r] = q.z; v0[r] = q.w;
+      } else {
+        v0[r] = ld_once(vol, c);
+      }
       xw0[r] = LX(c + dW); xe0[r] = LX(c + dE); dw0[r] = 0; de0[r] = 0;
       if constexpr (TR == 0) { dw0[r] = dut[c + dW]; de0[r] = dut[c + dE]; }
     }
@@ -464,8 +484,16 @@ __device__ __forceinline__ void march_rows(const KApply<CT, VX, VO, VU>& a, int 
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const long c = cb[r] + off, mU = up ? c + sz : c;
-        xp[r] = LX(mU); dp[r] = dut[mU]; vp[r] = ld_once(vol, mU);
-        kz0[r] = ld_once(kz, c); tz0[r] = ld_once(tz, c); gz0[r] = ld_once(gzz, c);
+        xp[r] = LX(mU); dp[r] = dut[mU];
+        if constexpr (ZP) {
+          const v4 q = __builtin_nontemporal_load(PKZ + mU);
+          vp[r] = q.w;
+          kz0[r] = kzc[r]; tz0[r] = tzc[r]; gz0[r] = gzc[r];
+          kzc[r] = q.x; tzc[r] = q.y; gzc[r] = q.z;
+        } else {
+          vp[r] = ld_once(vol, mU);
+          kz0[r] = ld_once(kz, c); tz0[r] = ld_once(tz, c); gz0[r] = ld_once(gzz, c);
+        }
         kyc[r] = ky[c]; tyc[r] = ty[c];
         xwp[r] = LX(mU + dW); xep[r] = LX(mU + dE); dwp[r] = 0; dep[r] = 0;
         if constexpr (TR == 0) { dwp[r] = dut[mU + dW]; dep[r] = dut[mU + dE]; }

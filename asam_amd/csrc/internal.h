@@ -31,6 +31,7 @@ struct Coef {
   T *tx = nullptr, *ty = nullptr, *tz = nullptr, *gz = nullptr;
   T *vol = nullptr, *dut = nullptr;
   T *pl = nullptr, *pt = nullptr, *pw = nullptr;
+  T *pkz = nullptr;  // FP32 2x2 blocks: (kz, tz, gz, vol) of each cell
 };
 
 // Launch-shape knobs (set by the autotuner or asam_sound_set_tuning).
@@ -51,7 +52,7 @@ struct Tuning {
                           // last workgroup; 2: the BiCGStab iteration's tails folded into
                           // the kernels that consume their scalars (0: a separate finalize
                           // launch each)
-  int apply_depth = 2;    // variant 3, FP32 2x2 blocks, two rows: levels of loads ahead (1, 2)
+  int apply_depth = 1;    // variant 3, FP32 2x2 blocks, two rows: levels of loads ahead (1, 2)
   int line_nt = 2;        // line solve's factor loads (and the vectors it reads once):
                           // 0 plain, 1 nontemporal, 2 nontemporal when the factors of the
                           // solve's precision exceed kLineNtBytes

@@ -267,6 +267,12 @@ __global__ void k_cvt(long n, const double* a, float* b) {
   if (i < n) b[i] = (float)a[i];
 }
 
+__global__ void k_pack_z(long n, Coef<float> c) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  reinterpret_cast<float4*>(c.pkz)[i] = make_float4(c.kz[i], c.tz[i], c.gz[i], c.vol[i]);
+}
+
 void launch_convert(hipStream_t s, const Geom& g, const Coef<double>& c64, Coef<float>& c32) {
   const long N = g.N, P = (long)g.np1 * g.np1 * fac_cells(g);
   const long PL = (long)lower_words(g.np1) * fac_cells(g);
@@ -277,6 +283,7 @@ void launch_convert(hipStream_t s, const Geom& g, const Coef<double>& c64, Coef<
   cv(c64.tx, c32.tx, N); cv(c64.ty, c32.ty, N); cv(c64.tz, c32.tz, N);
   cv(c64.gz, c32.gz, N); cv(c64.vol, c32.vol, N); cv(c64.dut, c32.dut, N);
   cv(c64.pl, c32.pl, PL); cv(c64.pt, c32.pt, P); cv(c64.pw, c32.pw, P);
+  if (c32.pkz) hipLaunchKernelGGL(k_pack_z, dim3((N + 255) / 256), dim3(256), 0, s, N, c32);
 }
 
 // Dead cells of the residual form: the apply's own test (march_tile), same operand order.

@@ -233,6 +233,7 @@ extern "C" int asam_sound_create(asam_sound_ctx** out, const asam_sound_grid* gr
     if (grid->np1 == 2) {
       r |= dalloc(c, &k.tx, N); r |= dalloc(c, &k.ty, N); r |= dalloc(c, &k.tz, N);
       r |= dalloc(c, &k.gz, N);
+      r |= dalloc(c, &k.pkz, 4 * N);
     }
     r |= dalloc(c, &k.vol, N); r |= dalloc(c, &k.dut, N);
     r |= dalloc(c, &k.pl, PL); r |= dalloc(c, &k.pt, P); r |= dalloc(c, &k.pw, P);

@@ -121,8 +121,9 @@ typedef struct {
                         factors of the solve's precision exceed 64 MiB, plain below, where
                         they stay resident in the Infinity Cache.  Same bits either way */
   int apply_depth;   /* variant 3 with FP32 coefficients, 2x2 blocks and two rows: the
-                        levels a thread loads ahead of the one it computes, 1 or 2
-                        (default).  Same bits */
+                        levels a thread loads ahead of the one it computes, 1 (default;
+                        the z-face coefficients and volumes then come from a packed copy,
+                        16 bytes per cell) or 2.  Same bits */
 } asam_sound_tuning;
 
 /* ---- context ---------------------------------------------------------- */
