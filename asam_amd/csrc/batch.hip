@@ -87,6 +87,7 @@ __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) 
     a.c.kx += M.off; a.c.ky += M.off; a.c.kz += M.off;
     a.c.vol += M.off; a.c.dut += M.off;
     if constexpr (NP == 2) { a.c.tx += M.off; a.c.ty += M.off; a.c.tz += M.off; a.c.gz += M.off; }
+    if (a.c.pkz) a.c.pkz += 4 * M.off;
     a.nx = M.nx; a.ny = M.ny; a.nz = M.nz;
     a.gx = M.agx; a.gy = M.agy; a.kp = M.akp; a.ntiles = M.ant;
     a.bxw = e.bxw; a.byw = 256 / e.bxw; a.xcd = 0;
@@ -111,7 +112,9 @@ __global__ __launch_bounds__(256) void k_ens_apply(KEnsApply<CT, VX, VO, VU> e) 
     a.done = nullptr;
     a.dead = e.dead ? e.dead + M.off : nullptr;
     double acc[2] = {0.0, 0.0};
-    if constexpr (RB == 2) march_rows<NP, 0, FL, 2, CT, VX, VO, VU>(a, g - M.at0, acc);
+    // FP32 2x2 blocks: z-face coefficients and volume from the packed pool
+    constexpr int ZP = NP == 2 && sizeof(CT) == 4 && (FL & AF_X0) == 0 ? 1 : 0;
+    if constexpr (RB == 2) march_rows<NP, 0, FL, 2, CT, VX, VO, VU, ZP>(a, g - M.at0, acc);
     else march_tile<NP, 0, FL, CT, VX, VO, VU>(a, g - M.at0, acc);
     if constexpr ((FL & (AF_YY | AF_YU)) != 0) {
       block_reduce_store<2>(acc, e.partial, g, tid, 256, e.pstride);
@@ -421,6 +424,7 @@ static Coef<T> member_coef(const Coef<T>& pool, const EnsMember& M, int np1) {
   Coef<T> k = pool;
   k.kx += M.off; k.ky += M.off; k.kz += M.off; k.vol += M.off; k.dut += M.off;
   if (np1 == 2) { k.tx += M.off; k.ty += M.off; k.tz += M.off; k.gz += M.off; }
+  if (k.pkz) k.pkz += 4 * M.off;
   const long fo = (long)np1 * np1 * M.foff;
   k.pl += lower_words(np1) * M.foff; k.pt += fo; k.pw += fo;
   return k;
@@ -816,6 +820,7 @@ extern "C" int asam_ens_create(asam_ens_ctx** out, int nmem, const asam_sound_gr
   };
   coef(c->c64);
   coef(c->c32);
+  if (np1 == 2) rc |= ealloc(c, &c->c32.pkz, 4 * N);  // packed (kz, tz, gz, vol), FP32 apply
   rc |= ealloc(c, &c->xb, c->n);
   float** v32[] = {&c->r, &c->rt, &c->p, &c->ph, &c->v, &c->rh, &c->t, &c->d};
   for (float** q : v32) rc |= ealloc(c, q, c->n);
