@@ -56,9 +56,14 @@ def parse():
     ap.add_argument("--cfg3-only", action="store_true",
                     help="measure only the 512x512x96 solves (tuned shapes, per-kernel "
                          "roofline) and exit (profiler runs)")
+    ap.add_argument("--solves-only", action="store_true",
+                    help="256x256x64: only the FP64 and mixed BiCGStab solves (no projection, "
+                         "FGMRES, varying-b, 512x512x96, CPU or transport legs; profiler runs)")
     ap.add_argument("--tuning", default=None, metavar="JSON",
-                    help="--cfg3-only: launch shapes per precision from this file (as the "
-                         "'autotuned' field prints them) instead of running the autotuner")
+                    help="launch shapes per precision from this file (as the 'autotuned' "
+                         "field prints them) instead of running the autotuner: the "
+                         "256x256x64 solves, or the 512x512x96 ones with --cfg3-only "
+                         "(profiler runs: the traces then hold only the tuned shapes)")
     return ap.parse_args()
 
 
@@ -621,14 +626,22 @@ def main():
     # Timed region: K solves, uninstrumented (value, ms_per_step); then the same K solves
     # with every kernel launch bracketed by HIP events (per-kernel durations and bytes for
     # the roofline).  Both runs issue identical launch sequences.
+    given = load_tuning(a.tuning) if a.tuning else (None, None)
+
+    def tune(tg):
+        if tg:
+            s.set_tuning(**tg)
+            return s.get_tuning()
+        return s.autotune(5)[0]
+
     s.set_control(precision="FP64")
-    tune64, tms64 = s.autotune(5)
+    tune64 = tune(given[0])
     t64, infos64, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
     t64_i, _, prof64 = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
     x64 = xbuf.download()
 
     s.set_control(precision="MIXED")
-    tunemx, tmsmx = s.autotune(5)
+    tunemx = tune(given[1])
     tmx, infosmx, _ = timed_solves(s, xbuf, bbuf, a.steps, a.warmup, a.tol, False)
     tmx_i, _, profmx = timed_solves(s, xbuf, bbuf, a.steps, 1, a.tol, True)
     xmx = xbuf.download()
@@ -640,9 +653,11 @@ def main():
     true_rel = float(np.linalg.norm(r) / np.linalg.norm(p.b))
     diff_vs_fp64_gpu = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
-    line_proj = projection_leg(s, p, max(2, a.steps // 4), a.tol)
-    fgm = fgmres_leg(s, xbuf, bbuf, max(2, a.steps // 4), a.tol)
-    vary = varying_b_leg(s, p, max(4, a.steps // 2), a.tol)
+    line_proj = fgm = vary = None
+    if not a.solves_only:
+        line_proj = projection_leg(s, p, max(2, a.steps // 4), a.tol)
+        fgm = fgmres_leg(s, xbuf, bbuf, max(2, a.steps // 4), a.tol)
+        vary = varying_b_leg(s, p, max(4, a.steps // 2), a.tol)
 
     roof, dom = roofline_from_profile(profmx, a.np1)
     if roof is not None:
@@ -682,6 +697,8 @@ def main():
         "kernels_fp64": class_table(prof64, a.steps),
     }
     s.close()
+    if a.solves_only:
+        a.no_hbm = a.no_cpu = a.no_transport = True
     if not a.no_hbm:
         line["roofline_cfg3"] = cfg3_leg(a.np1, a.tol, max(4, a.steps // 4), not a.no_cpu)
     if not a.no_cpu:
