@@ -40,6 +40,10 @@ def kernel_class(name: str):
                   name)
     if m:
         return f"blas_{'fp64' if m.group(2) == 'double' else 'fp32'}"
+    if "k_fill<float>" in name:
+        # the FP32 correction's zero fill at each refinement step, which the live profiler
+        # counts with the FP32 BLAS class
+        return "blas_fp32"
     if "k_finalize" in name:
         return "reduce"
     return None
