@@ -1510,8 +1510,9 @@ extern "C" int asam_sound_autotune(asam_sound_ctx* c, int reps, asam_sound_tunin
           if (rc) { c->g.tune = t0; return rc; }
           if (m < best_ms) { best_ms = m; bt = c->g.tune; }
         }
-  // budgets past 96 KiB stage every level of an FP64 2x2 column of up to 96 levels
-  const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48, 80, 100, 128};
+  // budgets past 96 KiB stage every level of an FP64 2x2 column of up to 96 levels; 76 KiB
+  // is the most that still lets two workgroups share a CU's 160 KiB
+  const int windows[] = {1, 4, 8}, ldss[] = {1, 0, 16, 32, 48, 76, 80, 100, 128};
   const Tuning ba = bt;
   for (int w : windows)
     for (int l : ldss) {

@@ -514,9 +514,10 @@ def test_row_blocked_apply_thin_grids(np1, dims, oracle_mod, sound_mod):
 @pytest.mark.parametrize("transpose", [False, True])
 def test_line_large_lds_budget(transpose, oracle_mod, sound_mod):
     """96-level FP64 2x2 columns staged entirely in LDS (96 KiB per workgroup, budgets 100
-    and 128 KiB) against the default 60 KiB budget (60 levels in LDS, 36 in the output
-    vector) and the oracle: FP64 solve to 1e-10 with the oracle's iteration count (+-1),
-    forward and transposed."""
+    and 128 KiB) and partly past 64 KiB (76 KiB: 76 levels in LDS, 20 in the output vector)
+    against the default 60 KiB budget (60 levels in LDS, 36 in the output vector) and the
+    oracle: FP64 solve to 1e-10 with the oracle's iteration count (+-1), forward and
+    transposed."""
     p = W.make_problem(64, 8, 96, 1000.0, 1000.0, 150.0, H=600.0, a=2500.0,
                        ridge_xc=32000.0, np1=2, grad_full=1, seed=5)
     orc = oracle_mod.Oracle(p)
@@ -525,7 +526,7 @@ def test_line_large_lds_budget(transpose, oracle_mod, sound_mod):
     s.set_control(precision="FP64")
     solve = s.SolveSound_T if transpose else s.SolveSound
     ref = None
-    for lds in (1, 100, 128):
+    for lds in (1, 76, 100, 128):
         s.set_tuning(line_lds=lds)
         x, mg, _, info = solve(np.zeros(p.n), p.b, 300, 1e-8)
         if ref is None:
