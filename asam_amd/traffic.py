@@ -61,7 +61,7 @@ def read_counter(dirname: str):
     return per
 
 
-def main():
+def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("fetch")
     ap.add_argument("write")
@@ -70,7 +70,9 @@ def main():
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     ap.add_argument("--workload", required=True)
     ap.add_argument("--command", required=True)
-    a = ap.parse_args()
+    ap.add_argument("--out", default=OUT, help="the JSON file to update (default: traffic.json "
+                                               "beside this module)")
+    a = ap.parse_args(argv)
     cf, cw = read_counter(a.calib_fetch), read_counter(a.calib_write)
     kib = a.copy_bytes / 1024.0
     fetch_scale = kib / (sum(cf["copy"]) / len(cf["copy"]))
@@ -90,13 +92,13 @@ def main():
                     "launches": len(f[cls])}
     # one entry per workload; the other workloads' entries are kept
     try:
-        with open(OUT) as fh:
+        with open(a.out) as fh:
             allw = json.load(fh)
     except (OSError, ValueError):
         allw = {}
     allw = {"workloads": dict(allw.get("workloads", {}))}
     allw["workloads"][a.workload] = out
-    with open(OUT, "w") as fh:
+    with open(a.out, "w") as fh:
         json.dump(allw, fh, indent=1)
     print(json.dumps(out, indent=1))
 
